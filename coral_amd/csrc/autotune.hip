@@ -1,0 +1,205 @@
+// Autotuner: per (routine, op pair, size bucket) tile choice, persisted as text.
+//
+// CORAL fixes its blocking at compile time (MC/NC/KC = 384/576/256 and MR x NR = 8 x 12
+// for SGEMM, coral-aarch64/src/level3/sgemm.rs:92-94, f32_packers.rs:1-2).  On the GPU
+// the best macro-tile depends on the problem shape relative to the 256 CUs, so the
+// choice is made per bucket at run time: a bucket is (ceil(log2 m), ceil(log2 n),
+// ceil(log2 k)).  Lookups fall back to a shape heuristic when no entry exists.
+#include "coral_common.hpp"
+
+#include <map>
+#include <mutex>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+namespace coral {
+
+static std::mutex g_tune_mu;
+static std::map<std::string, int> g_table;
+static thread_local int g_forced_tile = -1;
+
+static int bucket(int64_t x) {
+  int b = 0;
+  while (b < 62 && (int64_t(1) << b) < x) ++b;
+  return b;
+}
+
+static std::string key_of(const char *routine, int opa, int opb, int64_t m, int64_t n, int64_t k) {
+  char buf[160];
+  snprintf(buf, sizeof(buf), "%s %d %d %d %d %d", routine, opa, opb, bucket(m), bucket(n), bucket(k));
+  return buf;
+}
+
+int tile_count(const char *routine) {
+  if (!routine) return 0;
+  int g = gemm_menu_size(routine);
+  if (g) return g;
+  return batched_menu_size(routine);
+}
+
+int tuned_tile(const char *routine, int opa, int opb, int64_t m, int64_t n, int64_t k) {
+  if (g_forced_tile >= 0) return g_forced_tile;
+  static const char *env = getenv("CORAL_TILE");
+  if (env && *env) return atoi(env);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_table.find(key_of(routine, opa, opb, m, n, k));
+  return it == g_table.end() ? -1 : it->second;
+}
+
+struct ForcedTile {
+  explicit ForcedTile(int t) { g_forced_tile = t; }
+  ~ForcedTile() { g_forced_tile = -1; }
+};
+
+}  // namespace coral
+
+using namespace coral;
+
+extern "C" {
+
+int coral_autotune_get(const char *routine, int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k,
+                       int32_t *tile) {
+  if (!routine || tile_count(routine) == 0) return -1;
+  if (!tile) return -7;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_table.find(key_of(routine, op_a, op_b, m, n, k));
+  *tile = it == g_table.end() ? -1 : it->second;
+  return CORAL_OK;
+}
+
+int coral_autotune_set(const char *routine, int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k,
+                       int32_t tile) {
+  if (!routine || tile_count(routine) == 0) return -1;
+  if (tile < 0 || tile >= tile_count(routine)) return -7;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_table[key_of(routine, op_a, op_b, m, n, k)] = tile;
+  return CORAL_OK;
+}
+
+int coral_autotune_clear(void) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_table.clear();
+  return CORAL_OK;
+}
+
+int coral_autotune_save(const char *path) {
+  if (!path) return -1;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  FILE *f = fopen(path, "w");
+  if (!f) { set_last_error("cannot write %s", path); return CORAL_ERR_IO; }
+  fprintf(f, "# coral autotune v1: routine op_a op_b log2m log2n log2k tile\n");
+  for (auto &kv : g_table) fprintf(f, "%s %d\n", kv.first.c_str(), kv.second);
+  fclose(f);
+  return CORAL_OK;
+}
+
+int coral_autotune_load(const char *path) {
+  if (!path) return -1;
+  FILE *f = fopen(path, "r");
+  if (!f) { set_last_error("cannot read %s", path); return CORAL_ERR_IO; }
+  char line[256];
+  std::map<std::string, int> loaded;
+  while (fgets(line, sizeof(line), f)) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    char routine[64];
+    int opa, opb, bm, bn, bk, tile;
+    if (sscanf(line, "%63s %d %d %d %d %d %d", routine, &opa, &opb, &bm, &bn, &bk, &tile) != 7) {
+      fclose(f);
+      set_last_error("malformed autotune line: %s", line);
+      return CORAL_ERR_IO;
+    }
+    if (tile_count(routine) == 0 || tile < 0 || tile >= tile_count(routine)) continue;
+    char key[160];
+    snprintf(key, sizeof(key), "%s %d %d %d %d %d", routine, opa, opb, bm, bn, bk);
+    loaded[key] = tile;
+  }
+  fclose(f);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (auto &kv : loaded) g_table[kv.first] = kv.second;
+  return CORAL_OK;
+}
+
+// Times every tile of the routine's menu on (m, n, k) with seeded data and stores the
+// fastest for the bucket.  Real routines use one float/double per element, complex
+// routines two.
+int coral_autotune_run(const char *routine, int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k,
+                       int32_t *chosen_tile) {
+  if (!routine) return -1;
+  const int ntiles = tile_count(routine);
+  if (ntiles == 0 || strstr(routine, "batched")) return -1;
+  if (!valid_op(op_a)) return -2;
+  if (!valid_op(op_b)) return -3;
+  if (m <= 0) return -4;
+  if (n <= 0) return -5;
+  if (k <= 0) return -6;
+  CORAL_TRY(ensure_init());
+  const bool dbl = routine[0] == 'd' || routine[0] == 'z';
+  const bool cpx = routine[0] == 'c' || routine[0] == 'z';
+  const size_t es = (dbl ? 8 : 4) * (cpx ? 2 : 1);
+  const int64_t lda = op_a == CORAL_NO_TRANSPOSE ? m : k;
+  const int64_t ldb = op_b == CORAL_NO_TRANSPOSE ? k : n;
+  void *a = nullptr, *b = nullptr, *c = nullptr;
+  CORAL_HIP(hipMalloc(&a, es * (size_t)m * (size_t)k));
+  CORAL_HIP(hipMalloc(&b, es * (size_t)k * (size_t)n));
+  CORAL_HIP(hipMalloc(&c, es * (size_t)m * (size_t)n));
+  const int64_t mult = cpx ? 2 : 1;
+  if (dbl) {
+    coral_fill_random_f64((double *)a, mult * m * k, 11, 0, nullptr);
+    coral_fill_random_f64((double *)b, mult * k * n, 12, 0, nullptr);
+  } else {
+    coral_fill_random_f32((float *)a, mult * m * k, 11, 0, nullptr);
+    coral_fill_random_f32((float *)b, mult * k * n, 12, 0, nullptr);
+  }
+  hipEvent_t e0, e1;
+  CORAL_HIP(hipEventCreate(&e0));
+  CORAL_HIP(hipEventCreate(&e1));
+  const float one_f[2] = {1.f, 0.f}, zero_f[2] = {0.f, 0.f};
+  const double one_d[2] = {1.0, 0.0}, zero_d[2] = {0.0, 0.0};
+  auto run_once = [&]() -> int {
+    if (!strcmp(routine, "sgemm"))
+      return coral_sgemm(op_a, op_b, m, n, k, 1.f, (float *)a, lda, (float *)b, ldb, 0.f, (float *)c, m, nullptr);
+    if (!strcmp(routine, "dgemm"))
+      return coral_dgemm(op_a, op_b, m, n, k, 1.0, (double *)a, lda, (double *)b, ldb, 0.0, (double *)c, m, nullptr);
+    if (!strcmp(routine, "sgemm_bf16x3"))
+      return coral_sgemm_bf16x3(op_a, op_b, m, n, k, 1.f, (float *)a, lda, (float *)b, ldb, 0.f, (float *)c, m,
+                                nullptr);
+    if (!strcmp(routine, "cgemm"))
+      return coral_cgemm(op_a, op_b, m, n, k, one_f, (float *)a, lda, (float *)b, ldb, zero_f, (float *)c, m, nullptr);
+    if (!strcmp(routine, "zgemm"))
+      return coral_zgemm(op_a, op_b, m, n, k, one_d, (double *)a, lda, (double *)b, ldb, zero_d, (double *)c, m,
+                         nullptr);
+    return coral_cgemm_3m_bf16x3(op_a, op_b, m, n, k, one_f, (float *)a, lda, (float *)b, ldb, zero_f, (float *)c, m,
+                                 nullptr);
+  };
+  int best = -1;
+  float best_ms = 1e30f;
+  int status = CORAL_OK;
+  for (int t = 0; t < ntiles && status == CORAL_OK; ++t) {
+    ForcedTile ft(t);
+    status = run_once();  // warm-up (also sizes the workspace)
+    if (status != CORAL_OK) break;
+    CORAL_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < 3 && status == CORAL_OK; ++r) status = run_once();
+    CORAL_HIP(hipEventRecord(e1, nullptr));
+    CORAL_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CORAL_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) { best_ms = ms; best = t; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  (void)hipFree(c);
+  if (status != CORAL_OK) return status;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_table[key_of(routine, op_a, op_b, m, n, k)] = best;
+  }
+  if (chosen_tile) *chosen_tile = best;
+  return CORAL_OK;
+}
+
+}  // extern "C"

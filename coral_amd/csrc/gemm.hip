@@ -1,0 +1,836 @@
+// Level-3 GEMM for gfx950: s/d/c/zgemm plus the split-precision BF16x3 forms.
+//
+// Design (replaces CORAL's Goto/BLIS 5-loop, coral-aarch64/src/level3/sgemm_nn.rs:10-99,
+// its packers f32_packers.rs:115-333 and the 8x12 NEON microkernel
+// microkernel/f32_mrxnr.rs:99-325):
+//
+//  1. A pack kernel copies op(A) (m x k) and op(B) (k x n, stored by columns) into
+//     "fragment-ordered" tiles.  Transposition, conjugation, complex expansion,
+//     power-of-two scaling and the BF16 hi/lo split all happen here, so the MFMA
+//     kernel sees a single layout.  Tiles are zero padded, so the main loop has no
+//     bounds checks.
+//  2. The MFMA kernel streams packed tiles HBM -> registers -> LDS (two stages,
+//     one barrier per K tile) and feeds 16x16 MFMA tiles from LDS with one
+//     conflict-free ds_read_b128 per lane per fragment.
+//  3. The epilogue applies alpha/beta (never reading C when beta == 0, as
+//     sgemm_nn.rs:29-45 and f32_mrxnr.rs:164-176 do), row/column scales and, for
+//     complex 4M, pairs real/imaginary columns across neighbouring lanes.
+//
+// Fragment-ordered tile layout.  A tile covers R rows x BK k-values.  One
+// "fragment" is 16 rows x KF k-values = 64 lanes x 16 bytes:
+//     lane l holds row (l % 16), k-values (l / 16) * VE + t, t < VE,
+// where VE = 16 / sizeof(element) and KF = 4 * VE.  A tile is stored as
+// [kc < BK/KF][rg < R/16][plane < PL][lane < 64][VE].  The MFMA k index is
+// permutation invariant as long as A and B use the same (lane, t) -> k map, so
+// every 16-byte vector feeds VE 16x16x4 MFMAs (f32/f64) or one 16x16x32 BF16 MFMA.
+#include "coral_common.hpp"
+
+#include <algorithm>
+#include <string.h>
+
+namespace coral {
+namespace gemm {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));  // one 16-byte fragment vector
+
+// ---------------------------------------------------------------------------
+// element traits
+// ---------------------------------------------------------------------------
+struct TF64 {  // FP64 MFMA 16x16x4, DGEMM / ZGEMM
+  using T = double;
+  using acc_t = f64x4;
+  static constexpr int VE = 2, PL = 1, KF = 8;
+  static constexpr bool SPLIT = false;
+  static constexpr bool ROWS_INTERLEAVED = true;
+  __device__ static __forceinline__ void mma(acc_t &c, const v4u *a, const v4u *b) {
+    f64x2 av = __builtin_bit_cast(f64x2, a[0]);
+    f64x2 bv = __builtin_bit_cast(f64x2, b[0]);
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(av[0], bv[0], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(av[1], bv[1], c, 0, 0, 0);
+  }
+};
+
+struct TF32 {  // FP32 MFMA 16x16x4, plain SGEMM / CGEMM 4M
+  using T = float;
+  using acc_t = f32x4;
+  static constexpr int VE = 4, PL = 1, KF = 16;
+  static constexpr bool SPLIT = false;
+  static constexpr bool ROWS_INTERLEAVED = false;
+  __device__ static __forceinline__ void mma(acc_t &c, const v4u *a, const v4u *b) {
+    f32x4 av = __builtin_bit_cast(f32x4, a[0]);
+    f32x4 bv = __builtin_bit_cast(f32x4, b[0]);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[0], bv[0], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[1], bv[1], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[2], bv[2], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[3], bv[3], c, 0, 0, 0);
+  }
+};
+
+struct TBF3 {  // BF16 hi/lo planes, three 16x16x32 BF16 MFMAs, FP32 accumulation
+  using T = float;
+  using acc_t = f32x4;
+  static constexpr int VE = 8, PL = 2, KF = 32;
+  static constexpr bool SPLIT = true;
+  static constexpr bool ROWS_INTERLEAVED = false;
+  __device__ static __forceinline__ void mma(acc_t &c, const v4u *a, const v4u *b) {
+    bf16x8 ah = __builtin_bit_cast(bf16x8, a[0]);
+    bf16x8 al = __builtin_bit_cast(bf16x8, a[1]);
+    bf16x8 bh = __builtin_bit_cast(bf16x8, b[0]);
+    bf16x8 bl = __builtin_bit_cast(bf16x8, b[1]);
+    // small cross terms first, then the dominant hi*hi product
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+  }
+};
+
+// IEEE binary32 -> bfloat16, round to nearest even (NaN stays a quiet NaN).
+__device__ __forceinline__ uint32_t bf16_rn_bits(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+__device__ __forceinline__ float bf16_bits_to_f32(uint32_t h) { return __uint_as_float(h << 16); }
+
+// ---------------------------------------------------------------------------
+// operand fetchers: value of the packed operand at (r, k) for batch slot z.
+// For A, r is a row of op(A); for B, r is a column of op(B).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct FetchReal {  // element p[r*sr + k*sk]
+  const T *p;
+  int64_t sr, sk;
+  __device__ __forceinline__ T operator()(int64_t r, int64_t k, int) const { return p[r * sr + k * sk]; }
+};
+
+// Complex 4M expansion of A: A' = [Re(opA) | Im(opA)], an R x 2K real matrix.
+template <typename T>
+struct FetchCplxA4M {
+  const T *p;
+  int64_t sr, sk, K;
+  bool conj;
+  __device__ __forceinline__ T operator()(int64_t r, int64_t kk, int) const {
+    bool im = kk >= K;
+    int64_t k = im ? kk - K : kk;
+    const T *e = p + 2 * (r * sr + k * sk);
+    return im ? (conj ? -e[1] : e[1]) : e[0];
+  }
+};
+
+// Complex 4M expansion of B: column 2j = [Re b_j ; -Im b_j], column 2j+1 = [Im b_j ; Re b_j],
+// so [Re(opA) | Im(opA)] * B' interleaves Re(C) and Im(C) column by column.
+template <typename T>
+struct FetchCplxB4M {
+  const T *p;
+  int64_t sr, sk, K;
+  bool conj;
+  __device__ __forceinline__ T operator()(int64_t jj, int64_t kk, int) const {
+    int64_t j = jj >> 1;
+    int part = (int)(jj & 1);
+    bool bot = kk >= K;
+    int64_t k = bot ? kk - K : kk;
+    const T *e = p + 2 * (j * sr + k * sk);
+    T re = e[0], im = conj ? -e[1] : e[1];
+    if (!bot) return part ? im : re;
+    return part ? re : -im;
+  }
+};
+
+// Complex 3M (Gauss) operands: z = 0 -> Re, z = 1 -> Im, z = 2 -> Re + Im.
+template <typename T>
+struct FetchCplx3M {
+  const T *p;
+  int64_t sr, sk;
+  bool conj;
+  __device__ __forceinline__ T operator()(int64_t r, int64_t k, int z) const {
+    const T *e = p + 2 * (r * sr + k * sk);
+    T re = e[0], im = conj ? -e[1] : e[1];
+    return z == 0 ? re : (z == 1 ? im : re + im);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// pack kernels
+// ---------------------------------------------------------------------------
+// Row scaling for the split form.  mx[r] holds the IEEE bits of max_k |f(r,k)|
+// (non-negative floats order like their bit patterns, so atomicMax on the bits is
+// an exact, order-independent max; NaN is skipped by fmaxf).  The row exponent is
+// e[r] = floor(log2(mx[r])), so row r scaled by 2^-e[r] has max magnitude in
+// [1, 2); zero or non-finite rows keep e = 0.
+__device__ __forceinline__ int exp_of_max(uint32_t bits) {
+  const float mx = __uint_as_float(bits);
+  return (mx > 0.f && mx <= 3.402823466e38f) ? ilogbf(mx) : 0;
+}
+
+__global__ void zero_u32_kernel(uint32_t *p, int64_t count) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// grid: x over rows, y over k chunks of `kchunk`, z over batch slots.
+template <class F>
+__global__ __launch_bounds__(256) void row_absmax_kernel(int64_t R, int64_t K, F f, uint32_t *mx, int64_t zstride,
+                                                         int kcontig, int64_t kchunk) {
+  const int z = blockIdx.z;
+  const int64_t k0 = (int64_t)blockIdx.y * kchunk;
+  const int64_t k1 = k0 + kchunk < K ? k0 + kchunk : K;
+  float m = 0.f;
+  int64_t r;
+  if (kcontig) {  // one wavefront per row, lanes walk k (coalesced)
+    const int lane = threadIdx.x & 63;
+    r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r < R)
+      for (int64_t k = k0 + lane; k < k1; k += 64) m = fmaxf(m, fabsf((float)f(r, k, z)));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if (lane != 0 || r >= R) return;
+  } else {  // one thread per row, neighbouring threads read neighbouring rows (coalesced)
+    r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    for (int64_t k = k0; k < k1; ++k) m = fmaxf(m, fabsf((float)f(r, k, z)));
+  }
+  atomicMax(&mx[z * zstride + r], __float_as_uint(m));
+}
+
+// One thread writes the PL vectors (16 bytes each) of one (tile, kc, rg, lane) unit.
+template <class TR, int BR, int BK, class F>
+__global__ void pack_kernel(v4u *__restrict__ dst, int64_t R, int64_t K, int64_t KT, int64_t units,
+                            F f, const uint32_t *__restrict__ exps, int64_t dst_zstride, int64_t exp_zstride) {
+  constexpr int VE = TR::VE, PL = TR::PL, KF = TR::KF, RG = BR / 16, KC = BK / KF;
+  constexpr int UNITS_PER_TILE = KC * RG * 64;
+  const int z = blockIdx.z;
+  int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= units) return;
+  int64_t tile = u / UNITS_PER_TILE;
+  int w = (int)(u % UNITS_PER_TILE);
+  int lane = w & 63;
+  int rest = w >> 6;
+  int rg = rest % RG;
+  int kc = rest / RG;
+  int64_t tr = tile / KT, tk = tile % KT;
+  int64_t r = tr * BR + rg * 16 + (lane & 15);
+  int64_t k0 = tk * BK + kc * KF + (lane >> 4) * VE;
+  v4u *out = dst + z * dst_zstride + tile * (int64_t)(UNITS_PER_TILE * PL) +
+               ((int64_t)(kc * RG + rg) * PL) * 64 + lane;
+  using T = typename TR::T;
+  if constexpr (!TR::SPLIT) {
+    T v[VE];
+#pragma unroll
+    for (int t = 0; t < VE; ++t) v[t] = (r < R && k0 + t < K) ? f(r, k0 + t, z) : T(0);
+    out[0] = __builtin_bit_cast(v4u, v);
+  } else {
+    int sc = 0;
+    if (r < R && exps) sc = -exp_of_max(exps[z * exp_zstride + r]);
+    uint32_t hi[VE], lo[VE];
+#pragma unroll
+    for (int t = 0; t < VE; ++t) {
+      float x = (r < R && k0 + t < K) ? (float)f(r, k0 + t, z) : 0.f;
+      x = ldexpf(x, sc);  // exact power-of-two scaling
+      uint32_t h = bf16_rn_bits(x);
+      hi[t] = h;
+      lo[t] = bf16_rn_bits(x - bf16_bits_to_f32(h));  // x - hi is exact in fp32
+    }
+    v4u vh, vl;
+    vh.x = hi[0] | (hi[1] << 16); vh.y = hi[2] | (hi[3] << 16);
+    vh.z = hi[4] | (hi[5] << 16); vh.w = hi[6] | (hi[7] << 16);
+    vl.x = lo[0] | (lo[1] << 16); vl.y = lo[2] | (lo[3] << 16);
+    vl.z = lo[4] | (lo[5] << 16); vl.w = lo[6] | (lo[7] << 16);
+    out[0] = vh;
+    out[64] = vl;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA kernel
+// ---------------------------------------------------------------------------
+enum Epi { EPI_REAL = 0, EPI_CPLX = 1, EPI_STORE_Z = 2 };
+
+struct GemmArgs {
+  const v4u *Ap, *Bp;
+  int64_t a_zstride, b_zstride;  // in 16-byte vectors
+  int64_t M, N;                  // output extent of the real product (N = 2n for complex 4M)
+  int64_t KT;                    // number of BK tiles
+  int tiles_m, tiles_n;
+  void *C;
+  int64_t ldc, c_zstride;
+  double ar, ai, br, bi;
+  int beta_zero;
+  const uint32_t *row_exp, *col_exp;  // split form only: max|.| bits per row of op(A) / column of op(B)
+  int64_t re_zstride, ce_zstride;
+};
+
+template <class TR, int BM, int BN, int BK, int EPI>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+  using T = typename TR::T;
+  using acc_t = typename TR::acc_t;
+  constexpr int PL = TR::PL, KF = TR::KF;
+  constexpr int KC = BK / KF;
+  constexpr int RGA = BM / 16, RGB = BN / 16;
+  constexpr int ATV = KC * RGA * PL * 64;  // vectors per A tile
+  constexpr int BTV = KC * RGB * PL * 64;
+  constexpr int LA = ATV / 256, LB = BTV / 256;
+  static_assert(ATV % 256 == 0 && BTV % 256 == 0, "tile must be a multiple of 4 KiB");
+  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+
+  extern __shared__ v4u lds[];  // [2][ATV + BTV]
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int z = blockIdx.z;
+
+  // tile id -> (tm, tn): spread consecutive ids over the 8 XCDs so neighbours
+  // land on one XCD, then walk column groups of 8 tile rows (L2 reuse of B panels).
+  int total = g.tiles_m * g.tiles_n;
+  int w = blockIdx.x;
+  if ((total & 7) == 0) w = (w & 7) * (total >> 3) + (w >> 3);
+  const int GROUP = 8;
+  int per_group = GROUP * g.tiles_n;
+  int grp = w / per_group;
+  int first_m = grp * GROUP;
+  int gsz = min(g.tiles_m - first_m, GROUP);
+  int win = w - grp * per_group;
+  int tm = first_m + win % gsz;
+  int tn = win / gsz;
+
+  const v4u *Ag = g.Ap + z * g.a_zstride + (int64_t)tm * g.KT * ATV + tid;
+  const v4u *Bg = g.Bp + z * g.b_zstride + (int64_t)tn * g.KT * BTV + tid;
+
+  acc_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
+
+  v4u ra[LA], rb[LB];
+#pragma unroll
+  for (int i = 0; i < LA; ++i) ra[i] = Ag[i * 256];
+#pragma unroll
+  for (int i = 0; i < LB; ++i) rb[i] = Bg[i * 256];
+#pragma unroll
+  for (int i = 0; i < LA; ++i) lds[tid + i * 256] = ra[i];
+#pragma unroll
+  for (int i = 0; i < LB; ++i) lds[ATV + tid + i * 256] = rb[i];
+  __syncthreads();
+
+  for (int64_t kt = 0; kt < g.KT; ++kt) {
+    const int cur = (int)(kt & 1);
+    const bool more = kt + 1 < g.KT;
+    if (more) {
+      const v4u *an = Ag + (kt + 1) * ATV;
+      const v4u *bn = Bg + (kt + 1) * BTV;
+#pragma unroll
+      for (int i = 0; i < LA; ++i) ra[i] = an[i * 256];
+#pragma unroll
+      for (int i = 0; i < LB; ++i) rb[i] = bn[i * 256];
+    }
+    const v4u *la = lds + cur * (ATV + BTV);
+    const v4u *lb = la + ATV;
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) {
+      v4u fa[FM][PL], fb[FN][PL];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int p = 0; p < PL; ++p) fa[i][p] = la[((kc * RGA + wm * FM + i) * PL + p) * 64 + lane];
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int p = 0; p < PL; ++p) fb[j][p] = lb[((kc * RGB + wn * FN + j) * PL + p) * 64 + lane];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) TR::mma(acc[i][j], fa[i], fb[j]);
+    }
+    if (more) {
+      v4u *dst = lds + (cur ^ 1) * (ATV + BTV);
+#pragma unroll
+      for (int i = 0; i < LA; ++i) dst[tid + i * 256] = ra[i];
+#pragma unroll
+      for (int i = 0; i < LB; ++i) dst[ATV + tid + i * 256] = rb[i];
+    }
+    __syncthreads();
+  }
+
+  // ---- epilogue ----
+  const T ar = (T)g.ar, ai = (T)g.ai, br = (T)g.br, bi = (T)g.bi;
+  T *C = reinterpret_cast<T *>(g.C);
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int64_t col = (int64_t)tn * BN + wn * WN + j * 16 + (lane & 15);
+      // 16x16 accumulator: lane l holds column l%16; register r holds row 4*(l/16)+r
+      // (f32/bf16 MFMA) or row l/16+4*r (f64 MFMA).
+      const int64_t row0 = (int64_t)tm * BM + wm * WM + i * 16 + (TR::ROWS_INTERLEAVED ? (lane >> 4) : 4 * (lane >> 4));
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = row0 + (TR::ROWS_INTERLEAVED ? 4 * r : r);
+        T v = acc[i][j][r];
+        if constexpr (EPI == EPI_CPLX) {
+          T pv = __shfl_xor(v, 1, 64);
+          const bool odd = lane & 1;
+          const T re = odd ? pv : v, im = odd ? v : pv;
+          const int64_t cj = col >> 1;
+          if (row < g.M && col < g.N) {
+            T *cp = C + 2 * (row + cj * g.ldc);
+            T ore = ar * re - ai * im, oim = ar * im + ai * re;
+            if (!g.beta_zero) {
+              T cr = cp[0], ci = cp[1];
+              ore += br * cr - bi * ci;
+              oim += br * ci + bi * cr;
+            }
+            cp[odd ? 1 : 0] = odd ? oim : ore;
+          }
+        } else {
+          if (row < g.M && col < g.N) {
+            if constexpr (TR::SPLIT) {
+              int sh = 0;
+              if (g.row_exp) sh += exp_of_max(g.row_exp[z * g.re_zstride + row]);
+              if (g.col_exp) sh += exp_of_max(g.col_exp[z * g.ce_zstride + col]);
+              v = ldexpf(v, sh);
+            }
+            if constexpr (EPI == EPI_STORE_Z) {
+              C[z * g.c_zstride + row + col * g.ldc] = v;
+            } else {
+              T *cp = C + row + col * g.ldc;
+              T o = ar * v;
+              if (!g.beta_zero) o += br * (*cp);
+              *cp = o;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// C := beta*C for the quick return (alpha == 0 or k == 0), never reading C when beta == 0.
+template <typename T, bool CPLX>
+__global__ void scale_c_kernel(T *C, int64_t m, int64_t n, int64_t ldc, double br, double bi) {
+  int64_t total = m * n;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = idx % m, j = idx / m;
+    if constexpr (CPLX) {
+      T *p = C + 2 * (i + j * ldc);
+      if (br == 0.0 && bi == 0.0) { p[0] = T(0); p[1] = T(0); }
+      else { T re = p[0], im = p[1]; p[0] = re * (T)br - im * (T)bi; p[1] = re * (T)bi + im * (T)br; }
+    } else {
+      T *p = C + i + j * ldc;
+      *p = (br == 0.0) ? T(0) : (T)br * (*p);
+    }
+  }
+}
+
+// 3M combine: Re C = T1 - T2, Im C = T3 - T1 - T2, then C := alpha*(Re + i Im) + beta*C.
+__global__ void combine_3m_kernel(const float *Tz, int64_t m, int64_t n, float *C, int64_t ldc,
+                                  float ar, float ai, float br, float bi, int beta_zero) {
+  int64_t total = m * n;
+  int64_t zs = m * n;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = idx % m, j = idx / m;
+    float t1 = Tz[idx], t2 = Tz[zs + idx], t3 = Tz[2 * zs + idx];
+    float re = t1 - t2, im = t3 - t1 - t2;
+    float *p = C + 2 * (i + j * ldc);
+    float ore = ar * re - ai * im, oim = ar * im + ai * re;
+    if (!beta_zero) {
+      float cr = p[0], ci = p[1];
+      ore += br * cr - bi * ci;
+      oim += br * ci + bi * cr;
+    }
+    p[0] = ore;
+    p[1] = oim;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct TileShape { int bm, bn, bk; };
+
+// Tile menus per element kind.  Index = tile id stored by the autotuner.
+template <int BM_, int BN_, int BK_> struct Shape { static constexpr int bm = BM_, bn = BN_, bk = BK_; };
+template <class... S> struct ShapeList {};
+
+// Tile menus per element kind; the index is the tile id the autotuner stores.
+using MenuF64 = ShapeList<Shape<128, 128, 16>, Shape<64, 64, 16>, Shape<128, 128, 32>>;
+using MenuF32 = ShapeList<Shape<128, 128, 32>, Shape<64, 64, 32>, Shape<256, 128, 32>, Shape<256, 256, 32>,
+                          Shape<128, 128, 64>>;
+using MenuBF3 = ShapeList<Shape<128, 128, 32>, Shape<64, 64, 32>, Shape<256, 128, 32>, Shape<256, 256, 32>,
+                          Shape<128, 128, 64>>;
+
+template <class L> struct MenuTable;
+template <class... S> struct MenuTable<ShapeList<S...>> {
+  static const TileShape *t() {
+    static const TileShape tab[] = {{S::bm, S::bn, S::bk}...};
+    return tab;
+  }
+  static constexpr int n = sizeof...(S);
+};
+
+template <class TR> struct Menu;
+template <> struct Menu<TF64> : MenuTable<MenuF64> { using list = MenuF64; };
+template <> struct Menu<TF32> : MenuTable<MenuF32> { using list = MenuF32; };
+template <> struct Menu<TBF3> : MenuTable<MenuBF3> { using list = MenuBF3; };
+
+template <class TR, int BM, int BN, int BK, int EPI>
+static int launch_gemm(const GemmArgs &g, int zdim, hipStream_t s) {
+  constexpr int KC = BK / TR::KF;
+  constexpr size_t lds_bytes = 2 * (size_t)KC * ((BM + BN) / 16) * TR::PL * 64 * 16;
+  auto kfn = gemm_kernel<TR, BM, BN, BK, EPI>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    CORAL_HIP(hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+    attr_set = true;
+  }
+  dim3 grid((unsigned)(g.tiles_m * g.tiles_n), 1, (unsigned)zdim);
+  const int slot = timing_begin(TK_GEMM, s);
+  hipLaunchKernelGGL(kfn, grid, dim3(256), lds_bytes, s, g);
+  CORAL_LAUNCH_CHECK();
+  timing_end(slot, s);
+  return CORAL_OK;
+}
+
+// instantiate the kernel for every menu entry and launch entry `idx`
+template <class TR, int EPI>
+static int dispatch_tile(int, const GemmArgs &, int, hipStream_t, ShapeList<>) {
+  set_last_error("gemm: tile id out of range");
+  return CORAL_ERR_UNSUPPORTED;
+}
+template <class TR, int EPI, class S, class... R>
+static int dispatch_tile(int idx, const GemmArgs &g, int zdim, hipStream_t s, ShapeList<S, R...>) {
+  if (idx == 0) return launch_gemm<TR, S::bm, S::bn, S::bk, EPI>(g, zdim, s);
+  return dispatch_tile<TR, EPI>(idx - 1, g, zdim, s, ShapeList<R...>{});
+}
+
+template <class TR, int EPI>
+static int launch_tile(int tile, const GemmArgs &g, int zdim, hipStream_t s) {
+  if (tile < 0 || tile >= Menu<TR>::n) tile = 0;
+  return dispatch_tile<TR, EPI>(tile, g, zdim, s, typename Menu<TR>::list{});
+}
+
+template <class TR, int BR, int BK, class F>
+static int launch_pack_t(v4u *dst, int64_t R, int64_t K, F f, const uint32_t *exps, int zdim,
+                         int64_t dst_zstride, int64_t exp_zstride, hipStream_t s) {
+  if constexpr (BK % TR::KF != 0) {
+    set_last_error("pack: k tile %d is not a multiple of %d", BK, TR::KF);
+    return CORAL_ERR_UNSUPPORTED;
+  } else {
+    int64_t RT = cdiv(R, BR), KT = cdiv(K, BK);
+    constexpr int UPT = (BK / TR::KF) * (BR / 16) * 64;
+    int64_t units = RT * KT * UPT;
+    dim3 grid((unsigned)cdiv(units, 256), 1, (unsigned)zdim);
+    hipLaunchKernelGGL((pack_kernel<TR, BR, BK, F>), grid, dim3(256), 0, s, dst, R, K, KT, units, f, exps,
+                       dst_zstride, exp_zstride);
+    CORAL_LAUNCH_CHECK();
+    return CORAL_OK;
+  }
+}
+
+template <class TR, class F>
+static int launch_pack(const TileShape &ts, bool is_a, v4u *dst, int64_t R, int64_t K, F f,
+                       const uint32_t *exps, int zdim, int64_t dst_zstride, int64_t exp_zstride, hipStream_t s) {
+  int br = is_a ? ts.bm : ts.bn;
+  if (ts.bk == 16) {
+    if (br == 64) return launch_pack_t<TR, 64, 16>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
+    if (br == 128) return launch_pack_t<TR, 128, 16>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
+    return launch_pack_t<TR, 256, 16>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
+  } else if (ts.bk == 32) {
+    if (br == 64) return launch_pack_t<TR, 64, 32>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
+    if (br == 128) return launch_pack_t<TR, 128, 32>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
+    return launch_pack_t<TR, 256, 32>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
+  } else {
+    if (br == 64) return launch_pack_t<TR, 64, 64>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
+    if (br == 128) return launch_pack_t<TR, 128, 64>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
+    return launch_pack_t<TR, 256, 64>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
+  }
+}
+
+template <class F>
+static int launch_row_max(int64_t R, int64_t K, F f, uint32_t *mx, int zdim, int64_t zstride, bool kcontig,
+                          hipStream_t s) {
+  // mx must hold zeros (see zero_u32_kernel) before the first launch
+  if (kcontig) {
+    const int64_t kchunk = 4096;
+    dim3 grid((unsigned)cdiv(R, 4), (unsigned)cdiv(K, kchunk), (unsigned)zdim);
+    hipLaunchKernelGGL(row_absmax_kernel<F>, grid, dim3(256), 0, s, R, K, f, mx, zstride, 1, kchunk);
+  } else {
+    const int64_t kchunk = 128;
+    dim3 grid((unsigned)cdiv(R, 256), (unsigned)cdiv(K, kchunk), (unsigned)zdim);
+    hipLaunchKernelGGL(row_absmax_kernel<F>, grid, dim3(256), 0, s, R, K, f, mx, zstride, 0, kchunk);
+  }
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+
+static int zero_u32(uint32_t *p, int64_t count, hipStream_t s) {
+  hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)std::min<int64_t>(cdiv(count, 256), 1024)), dim3(256), 0, s, p,
+                     count);
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+
+template <class TR>
+static size_t packed_bytes(int64_t R, int64_t K, int br, int bk) {
+  int64_t Rp = cdiv(R, br) * br, Kp = cdiv(K, bk) * bk;
+  return (size_t)Rp * (size_t)Kp * (16 / TR::VE) * TR::PL;
+}
+
+// Default tile choice when the autotuner has no entry: large tiles only when the
+// grid still fills the chip.
+template <class TR>
+static int default_tile(int64_t M, int64_t N, int64_t K) {
+  (void)K;
+  const TileShape *t = Menu<TR>::t();
+  const int cus = num_cus();
+  auto tiles = [&](const TileShape &s) { return cdiv(M, s.bm) * cdiv(N, s.bn); };
+  if constexpr (std::is_same<TR, TF64>::value) {
+    // FP64 MFMA is slow per instruction; 64x64 tiles keep 4 waves per SIMD resident
+    // (measured 58.5 vs 47.9 TFLOP/s for 128x128 at 8192^3)
+    return 1;
+  } else if constexpr (std::is_same<TR, TBF3>::value) {
+    if (tiles(t[3]) >= cus) return 3;  // 256x256: fewest LDS bytes per MFMA
+  }
+  if (tiles(t[0]) >= cus) return 0;
+  return 1;
+}
+
+
+// Shared driver for the real and complex-4M products.
+//   kind: 0 = real, 1 = complex 4M
+template <class TR, class FA, class FB>
+static int run_packed(const TileShape &ts, int epi, int64_t M, int64_t N, int64_t K, FA fa, FB fb,
+                      const uint32_t *rexp, const uint32_t *cexp, void *C, int64_t ldc, double ar, double ai,
+                      double br, double bi, int tile, hipStream_t s) {
+  const int64_t KT = cdiv(K, ts.bk);
+  void *pa = nullptr, *pb = nullptr;
+  CORAL_TRY(workspace(s, WS_PACK_A, packed_bytes<TR>(M, K, ts.bm, ts.bk), &pa));
+  CORAL_TRY(workspace(s, WS_PACK_B, packed_bytes<TR>(N, K, ts.bn, ts.bk), &pb));
+  CORAL_TRY(launch_pack<TR>(ts, true, (v4u *)pa, M, K, fa, rexp, 1, 0, 0, s));
+  CORAL_TRY(launch_pack<TR>(ts, false, (v4u *)pb, N, K, fb, cexp, 1, 0, 0, s));
+  GemmArgs g{};
+  g.Ap = (const v4u *)pa;
+  g.Bp = (const v4u *)pb;
+  g.M = M;
+  g.N = N;
+  g.KT = KT;
+  g.tiles_m = (int)cdiv(M, ts.bm);
+  g.tiles_n = (int)cdiv(N, ts.bn);
+  g.C = C;
+  g.ldc = ldc;
+  g.ar = ar; g.ai = ai; g.br = br; g.bi = bi;
+  g.beta_zero = (br == 0.0 && bi == 0.0);
+  g.row_exp = rexp;
+  g.col_exp = cexp;
+  if (epi == EPI_CPLX) return launch_tile<TR, EPI_CPLX>(tile, g, 1, s);
+  return launch_tile<TR, EPI_REAL>(tile, g, 1, s);
+}
+
+template <typename T>
+static void strides_a(int op, int64_t lda, int64_t &sr, int64_t &sk) {
+  if (op == CORAL_NO_TRANSPOSE) { sr = 1; sk = lda; } else { sr = lda; sk = 1; }
+}
+template <typename T>
+static void strides_b(int op, int64_t ldb, int64_t &sr, int64_t &sk) {
+  // rows of the packed B operand are columns j of op(B)
+  if (op == CORAL_NO_TRANSPOSE) { sr = ldb; sk = 1; } else { sr = 1; sk = ldb; }
+}
+
+// argument validation shared by all gemm entry points (CORAL argument order:
+// op_a=1 op_b=2 m=3 n=4 k=5 alpha=6 a=7 lda=8 b=9 ldb=10 beta=11 c=12 ldc=13)
+static int check_gemm(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, const void *a,
+                      int64_t lda, const void *b, int64_t ldb, const void *c, int64_t ldc, bool need_ab) {
+  if (!valid_op(op_a)) return -1;
+  if (!valid_op(op_b)) return -2;
+  if (m < 0) return -3;
+  if (n < 0) return -4;
+  if (k < 0) return -5;
+  int64_t arows = (op_a == CORAL_NO_TRANSPOSE) ? m : k;
+  int64_t brows = (op_b == CORAL_NO_TRANSPOSE) ? k : n;
+  if (need_ab && !a) return -7;
+  if (lda < imax64(1, arows)) return -8;
+  if (need_ab && !b) return -9;
+  if (ldb < imax64(1, brows)) return -10;
+  if (!c && m > 0 && n > 0) return -12;
+  if (ldc < imax64(1, m)) return -13;
+  return CORAL_OK;
+}
+
+template <typename T, bool CPLX>
+static int quick_scale(T *c, int64_t m, int64_t n, int64_t ldc, double br, double bi, hipStream_t s) {
+  if (!CPLX && br == 1.0) return CORAL_OK;
+  if (CPLX && br == 1.0 && bi == 0.0) return CORAL_OK;
+  int64_t blocks = std::min<int64_t>(cdiv(m * n, 256), 4096);
+  hipLaunchKernelGGL((scale_c_kernel<T, CPLX>), dim3((unsigned)blocks), dim3(256), 0, s, c, m, n, ldc, br, bi);
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+
+template <class TR, typename T>
+static int real_gemm(const char *routine, int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, T alpha,
+                     const T *a, int64_t lda, const T *b, int64_t ldb, T beta, T *c, int64_t ldc,
+                     coral_stream_t stream) {
+  bool trivial = (alpha == T(0) || k == 0);
+  CORAL_TRY(check_gemm(op_a, op_b, m, n, k, a, lda, b, ldb, c, ldc, !trivial && m > 0 && n > 0));
+  if (m == 0 || n == 0) return CORAL_OK;
+  CORAL_TRY(ensure_init());
+  hipStream_t s = resolve(stream);
+  if (trivial) return quick_scale<T, false>(c, m, n, ldc, (double)beta, 0.0, s);
+  int tile = tuned_tile(routine, op_a, op_b, m, n, k);
+  if (tile < 0 || tile >= Menu<TR>::n) tile = default_tile<TR>(m, n, k);
+  const TileShape ts = Menu<TR>::t()[tile];
+  FetchReal<T> fa{a, 0, 0}, fb{b, 0, 0};
+  strides_a<T>(op_a, lda, fa.sr, fa.sk);
+  strides_b<T>(op_b, ldb, fb.sr, fb.sk);
+  if constexpr (TR::SPLIT) {
+    uint32_t *ex = nullptr;
+    CORAL_TRY(workspace(s, WS_AUX0, sizeof(uint32_t) * (size_t)(m + n), (void **)&ex));
+    CORAL_TRY(zero_u32(ex, m + n, s));
+    CORAL_TRY(launch_row_max(m, k, fa, ex, 1, 0, fa.sk == 1, s));
+    CORAL_TRY(launch_row_max(n, k, fb, ex + m, 1, 0, fb.sk == 1, s));
+    return run_packed<TR>(ts, EPI_REAL, m, n, k, fa, fb, ex, ex + m, c, ldc, (double)alpha, 0.0, (double)beta,
+                          0.0, tile, s);
+  } else {
+    return run_packed<TR>(ts, EPI_REAL, m, n, k, fa, fb, nullptr, nullptr, c, ldc, (double)alpha, 0.0,
+                          (double)beta, 0.0, tile, s);
+  }
+}
+
+template <class TR, typename T>
+static int complex_gemm_4m(const char *routine, int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k,
+                           const T *alpha, const T *a, int64_t lda, const T *b, int64_t ldb, const T *beta,
+                           T *c, int64_t ldc, coral_stream_t stream) {
+  if (!alpha) return -6;
+  if (!beta) return -11;
+  bool trivial = (alpha[0] == T(0) && alpha[1] == T(0)) || k == 0;
+  CORAL_TRY(check_gemm(op_a, op_b, m, n, k, a, lda, b, ldb, c, ldc, !trivial && m > 0 && n > 0));
+  if (m == 0 || n == 0) return CORAL_OK;
+  CORAL_TRY(ensure_init());
+  hipStream_t s = resolve(stream);
+  if (trivial) return quick_scale<T, true>(c, m, n, ldc, (double)beta[0], (double)beta[1], s);
+  // real product: (m) x (2n) x (2k)
+  int tile = tuned_tile(routine, op_a, op_b, m, n, k);
+  if (tile < 0 || tile >= Menu<TR>::n) tile = default_tile<TR>(m, 2 * n, 2 * k);
+  const TileShape ts = Menu<TR>::t()[tile];
+  FetchCplxA4M<T> fa{a, 0, 0, k, op_a == CORAL_CONJUGATE_TRANSPOSE};
+  FetchCplxB4M<T> fb{b, 0, 0, k, op_b == CORAL_CONJUGATE_TRANSPOSE};
+  strides_a<T>(op_a, lda, fa.sr, fa.sk);
+  strides_b<T>(op_b, ldb, fb.sr, fb.sk);
+  return run_packed<TR>(ts, EPI_CPLX, m, 2 * n, 2 * k, fa, fb, nullptr, nullptr, c, ldc, (double)alpha[0],
+                        (double)alpha[1], (double)beta[0], (double)beta[1], tile, s);
+}
+
+static int cgemm_3m(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, const float *alpha,
+                    const float *a, int64_t lda, const float *b, int64_t ldb, const float *beta, float *c,
+                    int64_t ldc, coral_stream_t stream) {
+  if (!alpha) return -6;
+  if (!beta) return -11;
+  bool trivial = (alpha[0] == 0.f && alpha[1] == 0.f) || k == 0;
+  CORAL_TRY(check_gemm(op_a, op_b, m, n, k, a, lda, b, ldb, c, ldc, !trivial && m > 0 && n > 0));
+  if (m == 0 || n == 0) return CORAL_OK;
+  CORAL_TRY(ensure_init());
+  hipStream_t s = resolve(stream);
+  if (trivial) return quick_scale<float, true>(c, m, n, ldc, beta[0], beta[1], s);
+  int tile = tuned_tile("cgemm_3m_bf16x3", op_a, op_b, m, n, k);
+  if (tile < 0 || tile >= Menu<TBF3>::n) tile = default_tile<TBF3>(m, n, k);
+  const TileShape ts = Menu<TBF3>::t()[tile];
+  FetchCplx3M<float> fa{a, 0, 0, op_a == CORAL_CONJUGATE_TRANSPOSE};
+  FetchCplx3M<float> fb{b, 0, 0, op_b == CORAL_CONJUGATE_TRANSPOSE};
+  strides_a<float>(op_a, lda, fa.sr, fa.sk);
+  strides_b<float>(op_b, ldb, fb.sr, fb.sk);
+  // exponents: 3 x m for A, 3 x n for B; products T_z (3 x m x n fp32)
+  uint32_t *ex = nullptr;
+  CORAL_TRY(workspace(s, WS_AUX0, sizeof(uint32_t) * 3 * (size_t)(m + n), (void **)&ex));
+  uint32_t *exa = ex, *exb = ex + 3 * m;
+  CORAL_TRY(zero_u32(ex, 3 * (m + n), s));
+  CORAL_TRY(launch_row_max(m, k, fa, exa, 3, m, fa.sk == 1, s));
+  CORAL_TRY(launch_row_max(n, k, fb, exb, 3, n, fb.sk == 1, s));
+  size_t abytes = packed_bytes<TBF3>(m, k, ts.bm, ts.bk), bbytes = packed_bytes<TBF3>(n, k, ts.bn, ts.bk);
+  void *pa = nullptr, *pb = nullptr, *tz = nullptr;
+  CORAL_TRY(workspace(s, WS_PACK_A, 3 * abytes, &pa));
+  CORAL_TRY(workspace(s, WS_PACK_B, 3 * bbytes, &pb));
+  CORAL_TRY(workspace(s, WS_AUX1, 3 * sizeof(float) * (size_t)m * (size_t)n, &tz));
+  CORAL_TRY(launch_pack<TBF3>(ts, true, (v4u *)pa, m, k, fa, exa, 3, (int64_t)(abytes / 16), m, s));
+  CORAL_TRY(launch_pack<TBF3>(ts, false, (v4u *)pb, n, k, fb, exb, 3, (int64_t)(bbytes / 16), n, s));
+  GemmArgs g{};
+  g.Ap = (const v4u *)pa;
+  g.Bp = (const v4u *)pb;
+  g.a_zstride = (int64_t)(abytes / 16);
+  g.b_zstride = (int64_t)(bbytes / 16);
+  g.M = m;
+  g.N = n;
+  g.KT = cdiv(k, ts.bk);
+  g.tiles_m = (int)cdiv(m, ts.bm);
+  g.tiles_n = (int)cdiv(n, ts.bn);
+  g.C = tz;
+  g.ldc = m;
+  g.c_zstride = m * n;
+  g.ar = 1.0;
+  g.beta_zero = 1;
+  g.row_exp = exa;
+  g.col_exp = exb;
+  g.re_zstride = m;
+  g.ce_zstride = n;
+  CORAL_TRY((launch_tile<TBF3, EPI_STORE_Z>(tile, g, 3, s)));
+  int64_t blocks = std::min<int64_t>(cdiv(m * n, 256), 8192);
+  hipLaunchKernelGGL(combine_3m_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (const float *)tz, m, n, c, ldc,
+                     alpha[0], alpha[1], beta[0], beta[1], (int)(beta[0] == 0.f && beta[1] == 0.f));
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+
+}  // namespace gemm
+
+int gemm_menu_size(const char *routine) {
+  if (!strcmp(routine, "dgemm") || !strcmp(routine, "zgemm")) return gemm::Menu<gemm::TF64>::n;
+  if (!strcmp(routine, "sgemm") || !strcmp(routine, "cgemm")) return gemm::Menu<gemm::TF32>::n;
+  if (!strcmp(routine, "sgemm_bf16x3") || !strcmp(routine, "cgemm_3m_bf16x3")) return gemm::Menu<gemm::TBF3>::n;
+  return 0;
+}
+
+}  // namespace coral
+
+using namespace coral;
+using namespace coral::gemm;
+
+extern "C" {
+
+int coral_sgemm(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, float alpha, const float *a,
+                int64_t lda, const float *b, int64_t ldb, float beta, float *c, int64_t ldc, coral_stream_t stream) {
+  return real_gemm<TF32, float>("sgemm", op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+}
+int coral_dgemm(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, double alpha, const double *a,
+                int64_t lda, const double *b, int64_t ldb, double beta, double *c, int64_t ldc,
+                coral_stream_t stream) {
+  return real_gemm<TF64, double>("dgemm", op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+}
+int coral_sgemm_bf16x3(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, float alpha, const float *a,
+                       int64_t lda, const float *b, int64_t ldb, float beta, float *c, int64_t ldc,
+                       coral_stream_t stream) {
+  return real_gemm<TBF3, float>("sgemm_bf16x3", op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+}
+int coral_cgemm(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, const float alpha[2], const float *a,
+                int64_t lda, const float *b, int64_t ldb, const float beta[2], float *c, int64_t ldc,
+                coral_stream_t stream) {
+  return complex_gemm_4m<TF32, float>("cgemm", op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+}
+int coral_zgemm(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, const double alpha[2],
+                const double *a, int64_t lda, const double *b, int64_t ldb, const double beta[2], double *c,
+                int64_t ldc, coral_stream_t stream) {
+  return complex_gemm_4m<TF64, double>("zgemm", op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+}
+int coral_cgemm_3m_bf16x3(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, const float alpha[2],
+                          const float *a, int64_t lda, const float *b, int64_t ldb, const float beta[2], float *c,
+                          int64_t ldc, coral_stream_t stream) {
+  return cgemm_3m(op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,254 @@
+// Level-2 GEMV for gfx950: y := alpha*op(A)*x + beta*y for s/d/c/z, op in {N, T, C}.
+//
+// CORAL streams A through the fused `?axpyf` (N, coral-aarch64/src/level1_special/daxpyf.rs:45)
+// and `?dotf` (T, ddotf.rs:37; C, cdotcf.rs:42) kernels after pre-scaling y by beta
+// (dgemv_notranspose.rs:88) and x by alpha (:93).  Here both variants are one pass of
+// HBM streaming over A with deterministic two-level reductions:
+//
+//  N: a workgroup owns 256*RPT consecutive rows x a chunk of columns; each thread
+//     keeps RPT row sums and reads 32 contiguous bytes of every column it visits
+//     (two 16-byte loads; four columns in flight).  Column chunks write partial
+//     row sums, which a second kernel adds in chunk order and combines with
+//     alpha/beta.
+//  T/C: one wavefront per column; lanes walk the column 32 bytes at a time and the
+//     dot product is finished with a DPP/shuffle butterfly.  Long columns are
+//     split into row chunks whose partial dots are reduced the same way.
+#include "coral_common.hpp"
+
+#include <algorithm>
+
+namespace coral {
+namespace l2 {
+
+struct alignas(16) Pack32 { uint4 a, b; };
+
+template <typename E, int RPT>
+__device__ __forceinline__ void load_run(E (&v)[RPT], const E *p, bool vec, int64_t avail) {
+  if (vec) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    Pack32 pk{q[0], q[1]};
+    __builtin_memcpy(&v[0], &pk, sizeof(E) * RPT);
+  } else {
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) v[r] = (r < avail) ? p[r] : ezero<E>();
+  }
+}
+
+template <typename E>
+__global__ __launch_bounds__(256) void gemv_n_kernel(int64_t m, int64_t n, const E *__restrict__ A, int64_t lda,
+                                                     const E *__restrict__ x, int64_t incx, E *__restrict__ P,
+                                                     int64_t cpc, int vec_ok) {
+  constexpr int RPT = 32 / sizeof(E);
+  const int64_t row0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * RPT;
+  if (row0 >= m) return;
+  const int64_t c0 = (int64_t)blockIdx.y * cpc;
+  const int64_t c1 = c0 + cpc < n ? c0 + cpc : n;
+  const bool full = vec_ok && row0 + RPT <= m;
+  const int64_t avail = m - row0;
+  E acc[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) acc[r] = ezero<E>();
+  const E *base = A + row0;
+  int64_t j = c0;
+  for (; j + 4 <= c1; j += 4) {
+    E a[4][RPT];
+    E xv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xv[u] = x[(j + u) * incx];
+      load_run<E, RPT>(a[u], base + (j + u) * lda, full, avail);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) acc[r] = emad(acc[r], a[u][r], xv[u]);
+  }
+  for (; j < c1; ++j) {
+    E a[RPT];
+    E xv = x[j * incx];
+    load_run<E, RPT>(a, base + j * lda, full, avail);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) acc[r] = emad(acc[r], a[r], xv);
+  }
+  E *out = P + (int64_t)blockIdx.y * m + row0;
+#pragma unroll
+  for (int r = 0; r < RPT; ++r)
+    if (r < avail) out[r] = acc[r];
+}
+
+template <typename E, bool CONJ>
+__global__ __launch_bounds__(256) void gemv_t_kernel(int64_t m, int64_t n, const E *__restrict__ A, int64_t lda,
+                                                     const E *__restrict__ x, int64_t incx, E *__restrict__ P,
+                                                     int64_t rpc, int vec_ok, int xvec_ok) {
+  constexpr int RPT = 32 / sizeof(E);
+  const int lane = threadIdx.x & 63;
+  const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= n) return;  // uniform per wavefront; no block-level barrier below
+  const int64_t r0 = (int64_t)blockIdx.y * rpc;
+  const int64_t r1 = r0 + rpc < m ? r0 + rpc : m;
+  const E *col = A + j * lda;
+  E acc = ezero<E>();
+  for (int64_t i = r0 + lane * RPT; i < r1; i += 64 * RPT) {
+    const int64_t avail = r1 - i;
+    const bool full = avail >= RPT;
+    E a[RPT], xv[RPT];
+    load_run<E, RPT>(a, col + i, full && vec_ok, avail);
+    if (incx == 1) {
+      load_run<E, RPT>(xv, x + i, full && xvec_ok, avail);
+    } else {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) xv[r] = (r < avail) ? x[(i + r) * incx] : ezero<E>();
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) acc = emad<CONJ>(acc, a[r], xv[r]);
+  }
+  acc = wave_esum(acc);
+  if (lane == 0) P[(int64_t)blockIdx.y * n + j] = acc;
+}
+
+// y[i] := alpha * sum_c P[c][i] + beta * y[i]   (chunks added in order)
+template <typename E>
+__global__ void gemv_reduce_kernel(int64_t len, int64_t nchunks, const E *__restrict__ P, E alpha, E beta,
+                                   int beta_zero, E *y, int64_t incy) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  E s = ezero<E>();
+  for (int64_t c = 0; c < nchunks; ++c) s = eadd(s, P[c * len + i]);
+  E *yp = y + i * incy;
+  *yp = axpby(alpha, s, beta, yp, beta_zero);
+}
+
+// y := beta*y (alpha == 0 path); beta == 0 stores zeros without reading y.
+template <typename E>
+__global__ void beta_only_kernel(int64_t len, E beta, int beta_zero, E *y, int64_t incy) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+    E *yp = y + i * incy;
+    *yp = beta_zero ? ezero<E>() : emul(beta, *yp);
+  }
+}
+
+template <typename E>
+static bool is_one(E v) {
+  if constexpr (elem_traits<E>::cplx) return v.x == 1 && v.y == 0;
+  else return v == 1;
+}
+template <typename E>
+static bool is_zero_h(E v) {
+  if constexpr (elem_traits<E>::cplx) return v.x == 0 && v.y == 0;
+  else return v == 0;
+}
+
+template <typename E>
+int launch_beta_only(int64_t len, E beta, E *y, int64_t incy, hipStream_t s) {
+  if (is_one(beta)) return CORAL_OK;
+  int64_t blocks = std::min<int64_t>(cdiv(len, 256), 1024);
+  hipLaunchKernelGGL(beta_only_kernel<E>, dim3((unsigned)blocks), dim3(256), 0, s, len, beta, (int)is_zero_h(beta), y,
+                     incy);
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+
+// CORAL argument order: trans=1 m=2 n=3 alpha=4 a=5 lda=6 x=7 incx=8 beta=9 y=10 incy=11
+template <typename E>
+int gemv_impl(int32_t trans, int64_t m, int64_t n, E alpha, const E *A, int64_t lda, const E *x, int64_t incx,
+              E beta, E *y, int64_t incy, coral_stream_t stream) {
+  if (!valid_op(trans)) return -1;
+  if (m < 0) return -2;
+  if (n < 0) return -3;
+  if (lda < imax64(1, m)) return -6;
+  if (incx <= 0) return -8;
+  if (incy <= 0) return -11;
+  // quick return, coral-aarch64/src/level2/dgemv_notranspose.rs:75-76
+  if (m == 0 || n == 0) return CORAL_OK;
+  if (is_zero_h(alpha) && is_one(beta)) return CORAL_OK;
+  const bool notrans = trans == CORAL_NO_TRANSPOSE;
+  const int64_t ylen = notrans ? m : n;
+  const int64_t xlen = notrans ? n : m;
+  if (!y) return -10;
+  CORAL_TRY(ensure_init());
+  hipStream_t s = resolve(stream);
+  if (is_zero_h(alpha)) return launch_beta_only<E>(ylen, beta, y, incy, s);
+  if (!A) return -5;
+  if (!x) return -7;
+  (void)xlen;
+  constexpr int RPT = 32 / sizeof(E);
+  const int cus = num_cus();
+  const bool a_aligned = ((uintptr_t)A % 16 == 0) && ((lda * (int64_t)sizeof(E)) % 16 == 0);
+  if (notrans) {
+    const int64_t row_blocks = cdiv(m, 256 * RPT);
+    // enough column chunks for ~8 workgroups per CU, each chunk >= 32 columns
+    int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(cdiv(8 * cus, row_blocks), cdiv(n, 32)));
+    const int64_t cpc = cdiv(n, chunks);
+    chunks = cdiv(n, cpc);
+    void *P = nullptr;
+    CORAL_TRY(workspace(s, WS_AUX2, sizeof(E) * (size_t)chunks * (size_t)m, &P));
+    const int slot = timing_begin(TK_GEMV, s);
+    hipLaunchKernelGGL(gemv_n_kernel<E>, dim3((unsigned)row_blocks, (unsigned)chunks), dim3(256), 0, s, m, n, A, lda,
+                       x, incx, (E *)P, cpc, (int)a_aligned);
+    CORAL_LAUNCH_CHECK();
+    timing_end(slot, s);
+    hipLaunchKernelGGL(gemv_reduce_kernel<E>, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, s, m, chunks,
+                       (const E *)P, alpha, beta, (int)is_zero_h(beta), y, incy);
+    CORAL_LAUNCH_CHECK();
+  } else {
+    const int64_t col_blocks = cdiv(n, 4);
+    const int64_t unit = 64 * RPT;
+    int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(cdiv(8 * cus, col_blocks), cdiv(m, 4 * unit)));
+    const int64_t rpc = cdiv(cdiv(m, chunks), unit) * unit;
+    chunks = cdiv(m, rpc);
+    const bool x_aligned = ((uintptr_t)x % 16 == 0);
+    void *P = nullptr;
+    CORAL_TRY(workspace(s, WS_AUX2, sizeof(E) * (size_t)chunks * (size_t)n, &P));
+    const int slot = timing_begin(TK_GEMV, s);
+    if (trans == CORAL_CONJUGATE_TRANSPOSE && elem_traits<E>::cplx)
+      hipLaunchKernelGGL((gemv_t_kernel<E, true>), dim3((unsigned)col_blocks, (unsigned)chunks), dim3(256), 0, s, m, n,
+                         A, lda, x, incx, (E *)P, rpc, (int)a_aligned, (int)x_aligned);
+    else
+      hipLaunchKernelGGL((gemv_t_kernel<E, false>), dim3((unsigned)col_blocks, (unsigned)chunks), dim3(256), 0, s, m,
+                         n, A, lda, x, incx, (E *)P, rpc, (int)a_aligned, (int)x_aligned);
+    CORAL_LAUNCH_CHECK();
+    timing_end(slot, s);
+    hipLaunchKernelGGL(gemv_reduce_kernel<E>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, n, chunks,
+                       (const E *)P, alpha, beta, (int)is_zero_h(beta), y, incy);
+    CORAL_LAUNCH_CHECK();
+  }
+  return CORAL_OK;
+}
+
+template int launch_beta_only<float>(int64_t, float, float *, int64_t, hipStream_t);
+template int launch_beta_only<double>(int64_t, double, double *, int64_t, hipStream_t);
+template int launch_beta_only<float2>(int64_t, float2, float2 *, int64_t, hipStream_t);
+template int launch_beta_only<double2>(int64_t, double2, double2 *, int64_t, hipStream_t);
+
+}  // namespace l2
+}  // namespace coral
+
+using namespace coral;
+using namespace coral::l2;
+
+extern "C" {
+
+int coral_sgemv(int32_t trans, int64_t m, int64_t n, float alpha, const float *a, int64_t lda, const float *x,
+                int64_t incx, float beta, float *y, int64_t incy, coral_stream_t stream) {
+  return gemv_impl<float>(trans, m, n, alpha, a, lda, x, incx, beta, y, incy, stream);
+}
+int coral_dgemv(int32_t trans, int64_t m, int64_t n, double alpha, const double *a, int64_t lda, const double *x,
+                int64_t incx, double beta, double *y, int64_t incy, coral_stream_t stream) {
+  return gemv_impl<double>(trans, m, n, alpha, a, lda, x, incx, beta, y, incy, stream);
+}
+int coral_cgemv(int32_t trans, int64_t m, int64_t n, const float alpha[2], const float *a, int64_t lda,
+                const float *x, int64_t incx, const float beta[2], float *y, int64_t incy, coral_stream_t stream) {
+  if (!alpha) return -4;
+  if (!beta) return -9;
+  return gemv_impl<float2>(trans, m, n, make_float2(alpha[0], alpha[1]), (const float2 *)a, lda, (const float2 *)x,
+                           incx, make_float2(beta[0], beta[1]), (float2 *)y, incy, stream);
+}
+int coral_zgemv(int32_t trans, int64_t m, int64_t n, const double alpha[2], const double *a, int64_t lda,
+                const double *x, int64_t incx, const double beta[2], double *y, int64_t incy, coral_stream_t stream) {
+  if (!alpha) return -4;
+  if (!beta) return -9;
+  return gemv_impl<double2>(trans, m, n, make_double2(alpha[0], alpha[1]), (const double2 *)a, lda,
+                            (const double2 *)x, incx, make_double2(beta[0], beta[1]), (double2 *)y, incy, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,413 @@
+// Level-1 BLAS for gfx950: dot/dotu/dotc, axpy, nrm2, i?amax, scal.
+//
+// Reductions are deterministic: a fixed grid (a function of n only) writes one
+// partial per workgroup, and a single workgroup combines the partials in a fixed
+// tree order.  Results stay in device memory (stream ordered).
+#include "coral_common.hpp"
+
+#include <algorithm>
+
+namespace coral {
+namespace l1 {
+
+constexpr int kThreads = 256;
+
+static int64_t reduce_grid(int64_t n) { return std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kThreads * 8), 1024)); }
+
+// block-wide sum of one value per thread (256 threads = 4 wavefronts)
+template <typename A>
+__device__ __forceinline__ A block_sum(A v, A *sh) {
+  v = wave_esum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  A r = ezero<A>();
+  if (threadIdx.x == 0) {
+    r = eadd(eadd(sh[0], sh[1]), eadd(sh[2], sh[3]));
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// dot: sum_i op(x_i) * y_i, op = conj for dotc
+// (coral-aarch64/src/level1/sdot.rs:42, ddot.rs:43, cdotu.rs:46, cdotc.rs:46)
+// ---------------------------------------------------------------------------
+template <typename E, bool CONJ>
+__global__ __launch_bounds__(kThreads) void dot_partial_kernel(int64_t n, const E *__restrict__ x, int64_t incx,
+                                                              const E *__restrict__ y, int64_t incy, E *part) {
+  __shared__ E sh[4];
+  E acc = ezero<E>();
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    acc = emad<CONJ>(acc, x[i * incx], y[i * incy]);
+  E r = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+template <typename E>
+__global__ __launch_bounds__(kThreads) void sum_final_kernel(const E *part, int64_t np, E *result) {
+  __shared__ E sh[4];
+  E acc = ezero<E>();
+  for (int64_t i = threadIdx.x; i < np; i += kThreads) acc = eadd(acc, part[i]);
+  E r = block_sum(acc, sh);
+  if (threadIdx.x == 0) *result = r;
+}
+
+template <typename E>
+__global__ void store_zero_kernel(E *result) { *result = ezero<E>(); }
+
+template <typename E, bool CONJ>
+static int dot_impl(int64_t n, const E *x, int64_t incx, const E *y, int64_t incy, E *result, coral_stream_t stream) {
+  // CORAL argument order: n=1 x=2 incx=3 y=4 incy=5 (result=6)
+  if (n < 0) return -1;
+  if (incx <= 0) return -3;
+  if (incy <= 0) return -5;
+  if (!result) return -6;
+  if (n > 0 && !x) return -2;
+  if (n > 0 && !y) return -4;
+  CORAL_TRY(ensure_init());
+  hipStream_t s = resolve(stream);
+  if (n == 0) {  // coral-aarch64/src/level1/sdot.rs: n == 0 returns 0
+    hipLaunchKernelGGL(store_zero_kernel<E>, dim3(1), dim3(1), 0, s, result);
+    CORAL_LAUNCH_CHECK();
+    return CORAL_OK;
+  }
+  const int64_t g = reduce_grid(n);
+  void *part = nullptr;
+  CORAL_TRY(workspace(s, WS_SMALL, sizeof(E) * (size_t)g, &part));
+  hipLaunchKernelGGL((dot_partial_kernel<E, CONJ>), dim3((unsigned)g), dim3(kThreads), 0, s, n, x, incx, y, incy,
+                     (E *)part);
+  CORAL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(sum_final_kernel<E>, dim3(1), dim3(kThreads), 0, s, (const E *)part, g, result);
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// axpy: y += alpha*x (coral-aarch64/src/level1/daxpy.rs:42-132); quick return when
+// n == 0 or alpha == 0 (:51-53)
+// ---------------------------------------------------------------------------
+template <typename E>
+__global__ __launch_bounds__(kThreads) void axpy_kernel(int64_t n, E alpha, const E *__restrict__ x, int64_t incx,
+                                                       E *__restrict__ y, int64_t incy) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    E *yp = y + i * incy;
+    *yp = emad(*yp, alpha, x[i * incx]);
+  }
+}
+
+template <typename E>
+static int axpy_impl(int64_t n, E alpha, const E *x, int64_t incx, E *y, int64_t incy, coral_stream_t stream) {
+  // n=1 alpha=2 x=3 incx=4 y=5 incy=6
+  if (n < 0) return -1;
+  if (incx <= 0) return -4;
+  if (incy <= 0) return -6;
+  if (n == 0 || eiszero_h(alpha)) return CORAL_OK;
+  if (!x) return -3;
+  if (!y) return -5;
+  CORAL_TRY(ensure_init());
+  const int64_t g = std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kThreads * 4), 8192));
+  hipLaunchKernelGGL(axpy_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, resolve(stream), n, alpha, x, incx, y,
+                     incy);
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// scal: x *= alpha (coral-aarch64/src/level1/dscal.rs:30-66 multiplies, so alpha == 0
+// keeps NaN/Inf as NaN)
+// ---------------------------------------------------------------------------
+template <typename E>
+__global__ __launch_bounds__(kThreads) void scal_kernel(int64_t n, E alpha, E *x, int64_t incx) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    E *p = x + i * incx;
+    *p = emul(*p, alpha);
+  }
+}
+
+template <typename E>
+static int scal_impl(int64_t n, E alpha, E *x, int64_t incx, coral_stream_t stream) {
+  if (n < 0) return -1;
+  if (incx <= 0) return -4;
+  if (n == 0) return CORAL_OK;
+  if (!x) return -3;
+  CORAL_TRY(ensure_init());
+  const int64_t g = std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kThreads * 4), 8192));
+  hipLaunchKernelGGL(scal_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, resolve(stream), n, alpha, x, incx);
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// nrm2.  Single precision: squares summed in double (no overflow/underflow is
+// possible for binary32 inputs), then sqrt and one rounding.  Double precision:
+// Blue's three-accumulator scaled sum of squares (one pass, no division; the
+// thresholds are those of Anderson, "A Fresh Look at Blue's Algorithm", 2017).
+// NaN anywhere gives NaN and Inf gives Inf (IEEE), where CORAL's chunked scaling
+// (coral-aarch64/src/level1/snrm2.rs:64-120, nrm2_helpers.rs:15-65) can skip a NaN.
+// ---------------------------------------------------------------------------
+struct Blue { double sml, med, big; };
+__device__ __forceinline__ Blue blue_add(Blue a, Blue b) { return Blue{a.sml + b.sml, a.med + b.med, a.big + b.big}; }
+
+constexpr double kTsml = 0x1.0p-511, kTbig = 0x1.0p+486, kSsml = 0x1.0p+537, kSbig = 0x1.0p-538;
+
+__device__ __forceinline__ void blue_acc(Blue &b, double v) {
+  double ax = fabs(v);
+  if (ax > kTbig) { double t = ax * kSbig; b.big = fma(t, t, b.big); }
+  else if (ax < kTsml) { double t = ax * kSsml; b.sml = fma(t, t, b.sml); }
+  else b.med = fma(ax, ax, b.med);
+}
+
+__device__ double blue_finish(Blue b) {
+  double scl, sumsq;
+  if (b.big > 0) {
+    if (b.med > 0 || b.med != b.med) b.big += (b.med * kSbig) * kSbig;
+    scl = 1.0 / kSbig;
+    sumsq = b.big;
+  } else if (b.sml > 0) {
+    if (b.med > 0 || b.med != b.med) {
+      double ymed = sqrt(b.med), ysml = sqrt(b.sml) / kSsml;
+      double ymin = ysml > ymed ? ymed : ysml, ymax = ysml > ymed ? ysml : ymed;
+      scl = 1.0;
+      double q = ymin / ymax;
+      sumsq = ymax * ymax * (1.0 + q * q);
+    } else {
+      scl = 1.0 / kSsml;
+      sumsq = b.sml;
+    }
+  } else {
+    scl = 1.0;
+    sumsq = b.med;
+  }
+  return scl * sqrt(sumsq);
+}
+
+template <typename E>
+__global__ __launch_bounds__(kThreads) void nrm2_partial_kernel(int64_t n, const E *__restrict__ x, int64_t incx,
+                                                               Blue *part) {
+  __shared__ Blue sh[kThreads];
+  Blue b{0, 0, 0};
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    E v = x[i * incx];
+    if constexpr (sizeof(typename elem_traits<E>::real) == 4) {  // binary32: plain double sum of squares
+      if constexpr (elem_traits<E>::cplx) { b.med = fma((double)v.x, (double)v.x, b.med); b.med = fma((double)v.y, (double)v.y, b.med); }
+      else b.med = fma((double)v, (double)v, b.med);
+    } else {
+      if constexpr (elem_traits<E>::cplx) { blue_acc(b, v.x); blue_acc(b, v.y); }
+      else blue_acc(b, v);
+    }
+  }
+  sh[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = blue_add(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void nrm2_final_kernel(const Blue *part, int64_t np, R *result, int single) {
+  __shared__ Blue sh[kThreads];
+  Blue b{0, 0, 0};
+  for (int64_t i = threadIdx.x; i < np; i += kThreads) b = blue_add(b, part[i]);
+  sh[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = blue_add(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *result = single ? (R)sqrt(sh[0].med) : (R)blue_finish(sh[0]);
+}
+
+template <typename E, typename R>
+static int nrm2_impl(int64_t n, const E *x, int64_t incx, R *result, coral_stream_t stream) {
+  // n=1 x=2 incx=3 (result=4)
+  if (n < 0) return -1;
+  if (incx <= 0) return -3;
+  if (!result) return -4;
+  if (n > 0 && !x) return -2;
+  CORAL_TRY(ensure_init());
+  hipStream_t s = resolve(stream);
+  if (n == 0) {
+    hipLaunchKernelGGL(store_zero_kernel<R>, dim3(1), dim3(1), 0, s, result);
+    CORAL_LAUNCH_CHECK();
+    return CORAL_OK;
+  }
+  const int64_t g = reduce_grid(n);
+  void *part = nullptr;
+  CORAL_TRY(workspace(s, WS_SMALL, sizeof(Blue) * (size_t)g, &part));
+  hipLaunchKernelGGL(nrm2_partial_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, s, n, x, incx, (Blue *)part);
+  CORAL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(nrm2_final_kernel<R>, dim3(1), dim3(kThreads), 0, s, (const Blue *)part, g, result,
+                     (int)(sizeof(R) == 4));
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// i?amax: 0-based index of the first maximum of |x_i| (complex: |re| + |im| in the
+// element precision), NaN never selected, n == 0 -> 0 (coral-aarch64/src/level1/
+// isamax.rs:49-55,72-76,96-147, icamax.rs:79-82).  The (value, index) reduction
+// keeps the larger value and, on equal values, the smaller index, which makes the
+// result independent of the reduction order.
+// ---------------------------------------------------------------------------
+struct ValIdx { double v; int64_t i; };
+
+__device__ __forceinline__ ValIdx vi_best(ValIdx a, ValIdx b) {
+  if (b.v > a.v || (b.v == a.v && b.i < a.i)) return b;
+  return a;
+}
+
+template <typename E>
+__device__ __forceinline__ double mag(E v) {
+  if constexpr (elem_traits<E>::cplx) {
+    typename elem_traits<E>::real m = fabs(v.x) + fabs(v.y);  // rounded in element precision
+    return (double)m;
+  } else {
+    return (double)fabs(v);
+  }
+}
+
+template <typename E>
+__global__ __launch_bounds__(kThreads) void iamax_partial_kernel(int64_t n, const E *__restrict__ x, int64_t incx,
+                                                                ValIdx *part) {
+  __shared__ ValIdx sh[kThreads];
+  ValIdx b{-1.0, INT64_MAX};  // -1: nothing seen (magnitudes are >= 0)
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    double m = mag(x[i * incx]);
+    if (m == m && m > b.v) { b.v = m; b.i = i; }  // strict '>' keeps the first index per thread
+  }
+  sh[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = vi_best(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(kThreads) void iamax_final_kernel(const ValIdx *part, int64_t np, int64_t *result) {
+  __shared__ ValIdx sh[kThreads];
+  ValIdx b{-1.0, INT64_MAX};
+  for (int64_t i = threadIdx.x; i < np; i += kThreads) b = vi_best(b, part[i]);
+  sh[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = vi_best(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *result = (sh[0].v < 0.0) ? 0 : sh[0].i;
+}
+
+template <typename E>
+static int iamax_impl(int64_t n, const E *x, int64_t incx, int64_t *result, coral_stream_t stream) {
+  // n=1 x=2 incx=3 (result=4).  CORAL returns 0 for n == 0 or incx <= 0 (isamax.rs:49)
+  if (n < 0) return -1;
+  if (!result) return -4;
+  CORAL_TRY(ensure_init());
+  hipStream_t s = resolve(stream);
+  if (n == 0 || incx <= 0) {
+    hipLaunchKernelGGL(store_zero_kernel<int64_t>, dim3(1), dim3(1), 0, s, result);
+    CORAL_LAUNCH_CHECK();
+    return CORAL_OK;
+  }
+  if (!x) return -2;
+  const int64_t g = reduce_grid(n);
+  void *part = nullptr;
+  CORAL_TRY(workspace(s, WS_SMALL, sizeof(ValIdx) * (size_t)g, &part));
+  hipLaunchKernelGGL(iamax_partial_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, s, n, x, incx, (ValIdx *)part);
+  CORAL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(iamax_final_kernel, dim3(1), dim3(kThreads), 0, s, (const ValIdx *)part, g, result);
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+
+}  // namespace l1
+}  // namespace coral
+
+using namespace coral;
+using namespace coral::l1;
+
+extern "C" {
+
+int coral_sdot(int64_t n, const float *x, int64_t incx, const float *y, int64_t incy, float *result, coral_stream_t s) {
+  return dot_impl<float, false>(n, x, incx, y, incy, result, s);
+}
+int coral_ddot(int64_t n, const double *x, int64_t incx, const double *y, int64_t incy, double *result,
+               coral_stream_t s) {
+  return dot_impl<double, false>(n, x, incx, y, incy, result, s);
+}
+int coral_cdotu(int64_t n, const float *x, int64_t incx, const float *y, int64_t incy, float *result, coral_stream_t s) {
+  return dot_impl<float2, false>(n, (const float2 *)x, incx, (const float2 *)y, incy, (float2 *)result, s);
+}
+int coral_cdotc(int64_t n, const float *x, int64_t incx, const float *y, int64_t incy, float *result, coral_stream_t s) {
+  return dot_impl<float2, true>(n, (const float2 *)x, incx, (const float2 *)y, incy, (float2 *)result, s);
+}
+int coral_zdotu(int64_t n, const double *x, int64_t incx, const double *y, int64_t incy, double *result,
+                coral_stream_t s) {
+  return dot_impl<double2, false>(n, (const double2 *)x, incx, (const double2 *)y, incy, (double2 *)result, s);
+}
+int coral_zdotc(int64_t n, const double *x, int64_t incx, const double *y, int64_t incy, double *result,
+                coral_stream_t s) {
+  return dot_impl<double2, true>(n, (const double2 *)x, incx, (const double2 *)y, incy, (double2 *)result, s);
+}
+
+int coral_saxpy(int64_t n, float alpha, const float *x, int64_t incx, float *y, int64_t incy, coral_stream_t s) {
+  return axpy_impl<float>(n, alpha, x, incx, y, incy, s);
+}
+int coral_daxpy(int64_t n, double alpha, const double *x, int64_t incx, double *y, int64_t incy, coral_stream_t s) {
+  return axpy_impl<double>(n, alpha, x, incx, y, incy, s);
+}
+int coral_caxpy(int64_t n, const float alpha[2], const float *x, int64_t incx, float *y, int64_t incy,
+                coral_stream_t s) {
+  if (!alpha) return -2;
+  return axpy_impl<float2>(n, make_float2(alpha[0], alpha[1]), (const float2 *)x, incx, (float2 *)y, incy, s);
+}
+int coral_zaxpy(int64_t n, const double alpha[2], const double *x, int64_t incx, double *y, int64_t incy,
+                coral_stream_t s) {
+  if (!alpha) return -2;
+  return axpy_impl<double2>(n, make_double2(alpha[0], alpha[1]), (const double2 *)x, incx, (double2 *)y, incy, s);
+}
+
+int coral_snrm2(int64_t n, const float *x, int64_t incx, float *result, coral_stream_t s) {
+  return nrm2_impl<float, float>(n, x, incx, result, s);
+}
+int coral_dnrm2(int64_t n, const double *x, int64_t incx, double *result, coral_stream_t s) {
+  return nrm2_impl<double, double>(n, x, incx, result, s);
+}
+int coral_scnrm2(int64_t n, const float *x, int64_t incx, float *result, coral_stream_t s) {
+  return nrm2_impl<float2, float>(n, (const float2 *)x, incx, result, s);
+}
+int coral_dznrm2(int64_t n, const double *x, int64_t incx, double *result, coral_stream_t s) {
+  return nrm2_impl<double2, double>(n, (const double2 *)x, incx, result, s);
+}
+
+int coral_isamax(int64_t n, const float *x, int64_t incx, int64_t *result, coral_stream_t s) {
+  return iamax_impl<float>(n, x, incx, result, s);
+}
+int coral_idamax(int64_t n, const double *x, int64_t incx, int64_t *result, coral_stream_t s) {
+  return iamax_impl<double>(n, x, incx, result, s);
+}
+int coral_icamax(int64_t n, const float *x, int64_t incx, int64_t *result, coral_stream_t s) {
+  return iamax_impl<float2>(n, (const float2 *)x, incx, result, s);
+}
+int coral_izamax(int64_t n, const double *x, int64_t incx, int64_t *result, coral_stream_t s) {
+  return iamax_impl<double2>(n, (const double2 *)x, incx, result, s);
+}
+
+int coral_sscal(int64_t n, float alpha, float *x, int64_t incx, coral_stream_t s) {
+  return scal_impl<float>(n, alpha, x, incx, s);
+}
+int coral_dscal(int64_t n, double alpha, double *x, int64_t incx, coral_stream_t s) {
+  return scal_impl<double>(n, alpha, x, incx, s);
+}
+int coral_cscal(int64_t n, const float alpha[2], float *x, int64_t incx, coral_stream_t s) {
+  if (!alpha) return -2;
+  return scal_impl<float2>(n, make_float2(alpha[0], alpha[1]), (float2 *)x, incx, s);
+}
+int coral_zscal(int64_t n, const double alpha[2], double *x, int64_t incx, coral_stream_t s) {
+  if (!alpha) return -2;
+  return scal_impl<double2>(n, make_double2(alpha[0], alpha[1]), (double2 *)x, incx, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,364 @@
+// Library runtime: device selection, error strings, per-stream workspaces,
+// memory/stream/event utilities and the seeded synthetic-data generators.
+#include "coral_common.hpp"
+
+#include <map>
+#include <mutex>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+namespace coral {
+
+static thread_local char g_last_error[512] = "";
+static std::mutex g_mu;
+static int g_device = -1;
+static int g_num_cus = 0;
+
+void set_last_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+  set_last_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+  if (e == hipErrorOutOfMemory) return CORAL_ERR_ALLOC;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return CORAL_ERR_NO_DEVICE;
+  return CORAL_ERR_HIP;
+}
+
+int ensure_init() {
+  if (g_device >= 0) return CORAL_OK;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_device >= 0) return CORAL_OK;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) {
+    set_last_error("no HIP device visible (%s)", hipGetErrorString(e));
+    return CORAL_ERR_NO_DEVICE;
+  }
+  int dev = 0;
+  CORAL_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  CORAL_HIP(hipGetDeviceProperties(&prop, dev));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_last_error("device %d is %s; libcoralhip is built for gfx950 only", dev, prop.gcnArchName);
+    return CORAL_ERR_NO_DEVICE;
+  }
+  g_num_cus = prop.multiProcessorCount;
+  g_device = dev;
+  return CORAL_OK;
+}
+
+int num_cus() { return g_num_cus > 0 ? g_num_cus : 256; }
+
+hipStream_t resolve(coral_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+struct WsEntry { void *ptr[WS_NSLOTS] = {}; size_t size[WS_NSLOTS] = {}; };
+static std::map<hipStream_t, WsEntry> g_ws;
+
+int workspace(hipStream_t stream, int slot, size_t bytes, void **ptr) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  WsEntry &w = g_ws[stream];
+  if (bytes == 0) bytes = 256;
+  if (w.size[slot] < bytes) {
+    if (w.ptr[slot]) {
+      // the old buffer may still be read by queued work on this stream
+      CORAL_HIP(hipStreamSynchronize(stream));
+      CORAL_HIP(hipFree(w.ptr[slot]));
+      w.ptr[slot] = nullptr;
+      w.size[slot] = 0;
+    }
+    size_t grow = bytes + bytes / 8;
+    grow = (grow + 4095) & ~size_t(4095);
+    void *p = nullptr;
+    CORAL_HIP(hipMalloc(&p, grow));
+    w.ptr[slot] = p;
+    w.size[slot] = grow;
+  }
+  *ptr = w.ptr[slot];
+  return CORAL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// kernel timing: event pairs recorded on the launch stream
+// ---------------------------------------------------------------------------
+struct TimedLaunch { int kind; hipEvent_t start, stop; };
+static bool g_timing_on = false;
+static std::vector<TimedLaunch> g_timed;
+static std::vector<hipEvent_t> g_event_pool;
+
+static hipEvent_t pool_event() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int timing_begin(int kind, hipStream_t s) {
+  if (!g_timing_on) return -1;
+  std::lock_guard<std::mutex> lk(g_mu);
+  TimedLaunch t{kind, pool_event(), pool_event()};
+  if (!t.start || !t.stop) return -1;
+  if (hipEventRecord(t.start, s) != hipSuccess) return -1;
+  g_timed.push_back(t);
+  return (int)g_timed.size() - 1;
+}
+
+void timing_end(int slot, hipStream_t s) {
+  if (slot < 0) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  (void)hipEventRecord(g_timed[slot].stop, s);
+}
+
+// ---------------------------------------------------------------------------
+// counter-based generator: h(seed, i) = mix(mix(seed + G) + (i + 1) * G)
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline uint64_t rng_at(uint64_t seed, uint64_t i) {
+  const uint64_t G = 0x9E3779B97F4A7C15ULL;
+  return mix64(mix64(seed + G) + (i + 1) * G);
+}
+
+template <typename T>
+__global__ void fill_random_kernel(T *dst, int64_t count, uint64_t seed, int kind) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) {
+    if (kind == 0) {
+      uint64_t h = rng_at(seed, (uint64_t)i);
+      if (sizeof(T) == 4) {
+        float u = (float)(h >> 40) * 0x1.0p-24f;
+        dst[i] = (T)(2.0f * u - 1.0f);
+      } else {
+        double u = (double)(h >> 11) * 0x1.0p-53;
+        dst[i] = (T)(2.0 * u - 1.0);
+      }
+    } else {
+      uint64_t h1 = rng_at(seed, 2 * (uint64_t)i), h2 = rng_at(seed, 2 * (uint64_t)i + 1);
+      double u1 = (double)((h1 >> 11) + 1) * 0x1.0p-53;
+      double u2 = (double)(h2 >> 11) * 0x1.0p-53;
+      double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2);
+      dst[i] = (T)z;
+    }
+  }
+}
+
+__global__ void scale_pow2_kernel(float *m, int64_t rows, int64_t cols, int64_t ld,
+                                  const int32_t *rexp, const int32_t *cexp) {
+  int64_t total = rows * cols;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < total; i += stride) {
+    int64_t r = i % rows, c = i / rows;
+    int e = (rexp ? rexp[r] : 0) + (cexp ? cexp[c] : 0);
+    float *p = m + r + c * ld;
+    *p = ldexpf(*p, e);
+  }
+}
+
+}  // namespace coral
+
+using namespace coral;
+
+extern "C" {
+
+int coral_init(void) { return ensure_init(); }
+
+const char *coral_status_string(int s) {
+  if (s < 0) return "invalid argument";
+  switch (s) {
+    case CORAL_OK: return "ok";
+    case CORAL_ERR_HIP: return "HIP runtime error";
+    case CORAL_ERR_NO_DEVICE: return "no usable gfx950 device";
+    case CORAL_ERR_ALLOC: return "device allocation failed";
+    case CORAL_ERR_UNSUPPORTED: return "unsupported argument combination";
+    case CORAL_ERR_IO: return "autotuner file i/o failed";
+    default: return "unknown status";
+  }
+}
+
+const char *coral_last_error(void) { return g_last_error; }
+
+int coral_device_name(char *buf, size_t len) {
+  CORAL_TRY(ensure_init());
+  hipDeviceProp_t prop;
+  CORAL_HIP(hipGetDeviceProperties(&prop, g_device));
+  snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  return CORAL_OK;
+}
+
+int coral_malloc(void **ptr, size_t bytes) {
+  CORAL_TRY(ensure_init());
+  CORAL_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+  return CORAL_OK;
+}
+int coral_free(void *ptr) {
+  if (!ptr) return CORAL_OK;
+  CORAL_HIP(hipFree(ptr));
+  return CORAL_OK;
+}
+int coral_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  CORAL_TRY(ensure_init());
+  CORAL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return CORAL_OK;
+}
+int coral_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  CORAL_TRY(ensure_init());
+  CORAL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return CORAL_OK;
+}
+int coral_memcpy_dtod(void *dst, const void *src, size_t bytes) {
+  CORAL_TRY(ensure_init());
+  CORAL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return CORAL_OK;
+}
+int coral_memset(void *dst, int value, size_t bytes) {
+  CORAL_TRY(ensure_init());
+  CORAL_HIP(hipMemset(dst, value, bytes));
+  return CORAL_OK;
+}
+int coral_synchronize(void) {
+  CORAL_TRY(ensure_init());
+  CORAL_HIP(hipDeviceSynchronize());
+  return CORAL_OK;
+}
+int coral_stream_create(coral_stream_t *stream) {
+  CORAL_TRY(ensure_init());
+  hipStream_t s;
+  CORAL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (coral_stream_t)s;
+  return CORAL_OK;
+}
+int coral_stream_destroy(coral_stream_t stream) {
+  if (!stream) return CORAL_OK;
+  CORAL_HIP(hipStreamSynchronize(resolve(stream)));
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_ws.find(resolve(stream));
+    if (it != g_ws.end()) {
+      for (int s = 0; s < WS_NSLOTS; ++s)
+        if (it->second.ptr[s]) (void)hipFree(it->second.ptr[s]);
+      g_ws.erase(it);
+    }
+  }
+  CORAL_HIP(hipStreamDestroy(resolve(stream)));
+  return CORAL_OK;
+}
+int coral_stream_synchronize(coral_stream_t stream) {
+  CORAL_TRY(ensure_init());
+  CORAL_HIP(hipStreamSynchronize(resolve(stream)));
+  return CORAL_OK;
+}
+int coral_event_create(void **event) {
+  CORAL_TRY(ensure_init());
+  hipEvent_t e;
+  CORAL_HIP(hipEventCreate(&e));
+  *event = (void *)e;
+  return CORAL_OK;
+}
+int coral_event_destroy(void *event) {
+  if (event) CORAL_HIP(hipEventDestroy((hipEvent_t)event));
+  return CORAL_OK;
+}
+int coral_event_record(void *event, coral_stream_t stream) {
+  CORAL_HIP(hipEventRecord((hipEvent_t)event, resolve(stream)));
+  return CORAL_OK;
+}
+int coral_event_elapsed_ms(float *ms, void *start, void *stop) {
+  CORAL_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+  return CORAL_OK;
+}
+int coral_event_synchronize(void *event) {
+  CORAL_HIP(hipEventSynchronize((hipEvent_t)event));
+  return CORAL_OK;
+}
+
+int coral_timing_enable(int32_t on) {
+  CORAL_TRY(ensure_init());
+  g_timing_on = on != 0;
+  return CORAL_OK;
+}
+
+int coral_timing_reset(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto &t : g_timed) {
+    (void)hipEventSynchronize(t.stop);
+    g_event_pool.push_back(t.start);
+    g_event_pool.push_back(t.stop);
+  }
+  g_timed.clear();
+  return CORAL_OK;
+}
+
+int coral_timing_read(int32_t kind, double *total_ms, int64_t *launches) {
+  if (kind < 0 || kind >= TK_NKINDS) return -1;
+  if (!total_ms) return -2;
+  if (!launches) return -3;
+  std::lock_guard<std::mutex> lk(g_mu);
+  double sum = 0.0;
+  int64_t cnt = 0;
+  for (auto &t : g_timed) {
+    if (t.kind != kind) continue;
+    CORAL_HIP(hipEventSynchronize(t.stop));
+    float ms = 0.f;
+    CORAL_HIP(hipEventElapsedTime(&ms, t.start, t.stop));
+    sum += ms;
+    ++cnt;
+  }
+  *total_ms = sum;
+  *launches = cnt;
+  return CORAL_OK;
+}
+
+int coral_fill_random_f32(float *dst, int64_t count, uint64_t seed, int32_t kind, coral_stream_t stream) {
+  if (!dst && count > 0) return -1;
+  if (count < 0) return -2;
+  if (kind != 0 && kind != 1) return -4;
+  CORAL_TRY(ensure_init());
+  if (count == 0) return CORAL_OK;
+  int64_t blocks = cdiv(count, 256);
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(fill_random_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, resolve(stream), dst, count, seed, kind);
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+int coral_fill_random_f64(double *dst, int64_t count, uint64_t seed, int32_t kind, coral_stream_t stream) {
+  if (!dst && count > 0) return -1;
+  if (count < 0) return -2;
+  if (kind != 0 && kind != 1) return -4;
+  CORAL_TRY(ensure_init());
+  if (count == 0) return CORAL_OK;
+  int64_t blocks = cdiv(count, 256);
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(fill_random_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, resolve(stream), dst, count, seed, kind);
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+int coral_scale_pow2_f32(float *m, int64_t rows, int64_t cols, int64_t ld, const int32_t *row_exp,
+                         const int32_t *col_exp, coral_stream_t stream) {
+  if (rows < 0) return -2;
+  if (cols < 0) return -3;
+  if (ld < imax64(1, rows)) return -4;
+  CORAL_TRY(ensure_init());
+  if (rows == 0 || cols == 0) return CORAL_OK;
+  int64_t blocks = cdiv(rows * cols, 256);
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(scale_pow2_kernel, dim3((unsigned)blocks), dim3(256), 0, resolve(stream), m, rows, cols, ld, row_exp, col_exp);
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,422 @@
+// Level-2 TRSV / TRMV for gfx950, plus batched DTRSV.
+//
+// CORAL solves op(A) x = b in NB = 8 diagonal blocks: a scalar substitution on the
+// diagonal block (coral-aarch64/src/level2/dtrlsv.rs:65-139) followed by a fused
+// `daxpyf` update of the rows below (:230-257, :316-347); transposed and upper
+// variants walk the blocks in the other direction (dtrlsv.rs:356-413, dtrusv.rs).
+// Complex ConjTranspose conjugates in the substitution and divides by conj(a_ii)
+// with den = re^2 + im^2 (ctrlsv.rs:189, 205-214).
+//
+// Here every variant is mapped onto one forward (lower) solve of a virtual matrix
+// M: index i of M is row i of op(A) for forward solves and row n-1-i for backward
+// solves, so M(i, c) (c <= i) reads A[.] through the op and the index reversal.
+// The solve is one persistent launch:
+//   * workgroups take row blocks of NB = 64 rows from an atomic ticket, so a
+//     workgroup only ever waits on blocks owned by workgroups already running;
+//   * a row block accumulates M(block, j) x_j over earlier blocks j as soon as
+//     block j publishes its flag (tile loads are issued before the wait);
+//   * the diagonal block is solved by one wavefront (lane = row), in the
+//     element precision (FP64 for D/Z), and published with a release fence.
+// TRMV (x := op(A) x) uses the same tiles without dependencies: one workgroup per
+// (row block, column block) pair writes a partial, a second kernel sums them.
+#include "coral_common.hpp"
+
+#include <algorithm>
+
+namespace coral {
+namespace tri {
+
+constexpr int NB = 64;
+
+template <typename E>
+struct TriView {
+  const E *A;
+  int64_t lda, n;
+  int trans;  // 0 N, 1 T, 2 C
+  int fwd;    // 1: forward (effective lower), 0: backward
+  __device__ __forceinline__ int64_t map(int64_t i) const { return fwd ? i : n - 1 - i; }
+  __device__ __forceinline__ E get(int64_t i, int64_t c) const {
+    const int64_t ri = map(i), rc = map(c);
+    if (trans == 0) return A[ri + rc * lda];
+    E v = A[rc + ri * lda];
+    return trans == 2 ? econj(v) : v;
+  }
+  // true when consecutive i are adjacent in memory (coalesced thread-per-row loads)
+  __device__ __forceinline__ bool row_contig() const { return trans == 0; }
+};
+
+// Loads tile (rows i0.., cols c0..) of M into registers, 16 elements per thread.
+// diag: only the strictly lower part (c < i) is read; the rest is zero.
+template <typename E>
+__device__ __forceinline__ void load_tile(const TriView<E> &v, int64_t i0, int64_t c0, bool diag, E (&reg)[16]) {
+  const int t = threadIdx.x;
+  const bool rc = v.row_contig();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    // row-contiguous: lanes walk rows; otherwise lanes walk columns
+    const int r = rc ? (t & 63) : ((t >> 6) + 4 * k);
+    const int c = rc ? ((t >> 6) + 4 * k) : (t & 63);
+    const int64_t i = i0 + r, cc = c0 + c;
+    const bool ok = i < v.n && cc < v.n && (!diag || c < r);
+    reg[k] = ok ? v.get(i, cc) : ezero<E>();
+  }
+}
+
+template <typename E>
+__device__ __forceinline__ void store_tile(const TriView<E> &v, E (*T)[NB + 1], const E (&reg)[16]) {
+  const int t = threadIdx.x;
+  const bool rc = v.row_contig();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int r = rc ? (t & 63) : ((t >> 6) + 4 * k);
+    const int c = rc ? ((t >> 6) + 4 * k) : (t & 63);
+    T[c][r] = reg[k];  // column-major tile: T[c][r] = M(i0 + r, c0 + c)
+  }
+}
+
+template <typename E>
+__device__ __forceinline__ E ediv(E a, E d) {
+  if constexpr (elem_traits<E>::cplx) {
+    // (a.x + i a.y) / (d.x + i d.y) = a * conj(d) / (d.x^2 + d.y^2)   (ctrlsv.rs:205-214)
+    auto den = d.x * d.x + d.y * d.y;
+    E r;
+    r.x = (a.x * d.x + a.y * d.y) / den;
+    r.y = (a.y * d.x - a.x * d.y) / den;
+    return r;
+  } else {
+    return a / d;
+  }
+}
+
+template <typename E>
+__device__ __forceinline__ E eshfl(E v, int src) {
+  if constexpr (elem_traits<E>::cplx) { v.x = __shfl(v.x, src, 64); v.y = __shfl(v.y, src, 64); return v; }
+  else return __shfl(v, src, 64);
+}
+template <typename E> __device__ __forceinline__ E eneg(E a) {
+  if constexpr (elem_traits<E>::cplx) { a.x = -a.x; a.y = -a.y; return a; } else return -a;
+}
+template <typename E> __device__ __forceinline__ E esub(E a, E b) { return eadd(a, eneg(b)); }
+
+// Wavefront 0 solves the diagonal block: rhs holds b_r - (off-diagonal sums) for lane r.
+// T holds the strictly lower part of the block; dvals the diagonal of M.
+template <typename E>
+__device__ __forceinline__ E solve_diag_block(E rhs, E (*T)[NB + 1], E dval, int unit, int nrows) {
+  const int lane = threadIdx.x & 63;
+  E xr = ezero<E>();
+  for (int c = 0; c < nrows; ++c) {
+    E xc_own = unit ? rhs : ediv(rhs, dval);
+    E xc = eshfl(xc_own, c);
+    if (lane == c) xr = xc;
+    if (lane > c) rhs = esub(rhs, emul(T[c][lane], xc));
+  }
+  return xr;
+}
+
+template <typename E>
+__global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t incx, int unit, int *ctrl, int nb) {
+  __shared__ E T[NB][NB + 1];
+  __shared__ E xs[NB];
+  __shared__ E part[4][NB];
+  __shared__ int bsh;
+  const int t = threadIdx.x, lane = t & 63, q = t >> 6;
+  if (t == 0) bsh = atomicAdd(&ctrl[0], 1);
+  __syncthreads();
+  const int b = bsh;
+  if (b >= nb) return;
+  int *flags = ctrl + 1;
+  const int64_t i0 = (int64_t)b * NB;
+  E acc = ezero<E>();
+  E reg[16];
+  if (b > 0) load_tile(v, i0, 0, false, reg);
+  for (int j = 0; j < b; ++j) {
+    // wait for x_j
+    if (t == 0) {
+      while (__hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __syncthreads();
+    __threadfence();
+    store_tile(v, T, reg);
+    if (t < NB) {
+      const int64_t c = (int64_t)j * NB + t;
+      xs[t] = c < v.n ? x[v.map(c) * incx] : ezero<E>();
+    }
+    __syncthreads();
+    if (j + 1 < b) load_tile(v, i0, (int64_t)(j + 1) * NB, false, reg);  // prefetch the next tile
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int c = q * 16 + k;
+      acc = emad(acc, T[c][lane], xs[c]);
+    }
+    __syncthreads();
+  }
+  part[q][lane] = acc;
+  // diagonal block
+  load_tile(v, i0, i0, true, reg);
+  store_tile(v, T, reg);
+  __syncthreads();
+  if (q == 0) {
+    const int64_t i = i0 + lane;
+    const int nrows = (int)std::min<int64_t>(NB, v.n - i0);
+    E rhs = ezero<E>(), dval = ezero<E>();
+    if (i < v.n) {
+      rhs = x[v.map(i) * incx];
+      rhs = esub(rhs, eadd(eadd(part[0][lane], part[1][lane]), eadd(part[2][lane], part[3][lane])));
+      if (!unit) dval = v.get(i, i);
+    }
+    E xr = solve_diag_block(rhs, T, dval, unit, nrows);
+    if (i < v.n) x[v.map(i) * incx] = xr;
+    __threadfence();
+  }
+  __syncthreads();
+  if (t == 0) __hip_atomic_store(&flags[b], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void zero_ints_kernel(int *p, int count) {
+  for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < count; i += blockDim.x * gridDim.x) p[i] = 0;
+}
+
+// ---- TRMV: partial y_b += M(b, j) x_j for j <= b ----
+__device__ __forceinline__ void pair_of(int64_t t, int64_t &b, int64_t &j) {
+  int64_t c = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while (c * (c + 1) / 2 > t) --c;
+  while ((c + 1) * (c + 2) / 2 <= t) ++c;
+  b = c;
+  j = t - c * (c + 1) / 2;
+}
+
+template <typename E>
+__global__ __launch_bounds__(256) void trmv_block_kernel(TriView<E> v, const E *x, int64_t incx, int unit, E *P,
+                                                         int64_t nb) {
+  __shared__ E T[NB][NB + 1];
+  __shared__ E xs[NB];
+  __shared__ E part[4][NB];
+  int64_t b, j;
+  pair_of(blockIdx.x, b, j);
+  const int t = threadIdx.x, lane = t & 63, q = t >> 6;
+  const int64_t i0 = b * NB, c0 = j * NB;
+  const bool diag = b == j;
+  E reg[16];
+  load_tile(v, i0, c0, diag, reg);
+  store_tile(v, T, reg);
+  if (t < NB) {
+    const int64_t c = c0 + t;
+    xs[t] = c < v.n ? x[v.map(c) * incx] : ezero<E>();
+  }
+  __syncthreads();
+  E acc = ezero<E>();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int c = q * 16 + k;
+    acc = emad(acc, T[c][lane], xs[c]);
+  }
+  part[q][lane] = acc;
+  __syncthreads();
+  if (q == 0) {
+    E s = eadd(eadd(part[0][lane], part[1][lane]), eadd(part[2][lane], part[3][lane]));
+    const int64_t i = i0 + lane;
+    if (diag && i < v.n) s = unit ? eadd(s, xs[lane]) : emad(s, v.get(i, i), xs[lane]);
+    P[(b * nb + j) * NB + lane] = s;
+  }
+}
+
+template <typename E>
+__global__ void trmv_reduce_kernel(TriView<E> v, E *x, int64_t incx, const E *P, int64_t nb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  const int64_t b = i / NB, w = i % NB;
+  E s = ezero<E>();
+  for (int64_t j = 0; j <= b; ++j) s = eadd(s, P[(b * nb + j) * NB + w]);
+  x[v.map(i) * incx] = s;
+}
+
+// CORAL argument order: uplo=1 trans=2 diag=3 n=4 a=5 lda=6 x=7 incx=8
+template <typename E>
+static int check_tr(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const E *a, int64_t lda, E *x,
+                    int64_t incx) {
+  if (!valid_uplo(uplo)) return -1;
+  if (!valid_op(trans)) return -2;
+  if (!valid_diag(diag)) return -3;
+  if (n < 0) return -4;
+  if (lda < imax64(1, n)) return -6;
+  if (incx <= 0) return -8;
+  if (n > 0 && !a) return -5;
+  if (n > 0 && !x) return -7;
+  return CORAL_OK;
+}
+
+template <typename E>
+static TriView<E> make_view(int32_t uplo, int32_t trans, const E *a, int64_t lda, int64_t n) {
+  TriView<E> v;
+  v.A = a;
+  v.lda = lda;
+  v.n = n;
+  // real routines map ConjugateTranspose to Transpose (coral-aarch64/src/level2/dtrlsv.rs:433)
+  v.trans = (trans == CORAL_CONJUGATE_TRANSPOSE && !elem_traits<E>::cplx) ? 1 : trans;
+  const bool lower = uplo == CORAL_LOWER_TRIANGULAR;
+  v.fwd = (lower == (trans == CORAL_NO_TRANSPOSE)) ? 1 : 0;
+  return v;
+}
+
+template <typename E>
+static int trsv_impl(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const E *a, int64_t lda, E *x,
+                     int64_t incx, coral_stream_t stream) {
+  CORAL_TRY(check_tr(uplo, trans, diag, n, a, lda, x, incx));
+  if (n == 0) return CORAL_OK;
+  CORAL_TRY(ensure_init());
+  hipStream_t s = resolve(stream);
+  const int nb = (int)cdiv(n, NB);
+  void *ctrl = nullptr;
+  CORAL_TRY(workspace(s, WS_SMALL, sizeof(int) * (size_t)(nb + 1), &ctrl));
+  hipLaunchKernelGGL(zero_ints_kernel, dim3((unsigned)std::min<int64_t>(cdiv(nb + 1, 256), 64)), dim3(256), 0, s,
+                     (int *)ctrl, nb + 1);
+  CORAL_LAUNCH_CHECK();
+  TriView<E> v = make_view(uplo, trans, a, lda, n);
+  hipLaunchKernelGGL(trsv_kernel<E>, dim3((unsigned)nb), dim3(256), 0, s, v, x, incx,
+                     (int)(diag == CORAL_UNIT_DIAGONAL), (int *)ctrl, nb);
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+
+template <typename E>
+static int trmv_impl(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const E *a, int64_t lda, E *x,
+                     int64_t incx, coral_stream_t stream) {
+  CORAL_TRY(check_tr(uplo, trans, diag, n, a, lda, x, incx));
+  if (n == 0) return CORAL_OK;
+  CORAL_TRY(ensure_init());
+  hipStream_t s = resolve(stream);
+  const int64_t nb = cdiv(n, NB);
+  void *P = nullptr;
+  CORAL_TRY(workspace(s, WS_AUX2, sizeof(E) * (size_t)(nb * nb * NB), &P));
+  TriView<E> v = make_view(uplo, trans, a, lda, n);
+  hipLaunchKernelGGL(trmv_block_kernel<E>, dim3((unsigned)(nb * (nb + 1) / 2)), dim3(256), 0, s, v, (const E *)x,
+                     incx, (int)(diag == CORAL_UNIT_DIAGONAL), (E *)P, nb);
+  CORAL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(trmv_reduce_kernel<E>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, v, x, incx,
+                     (const E *)P, nb);
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+
+// ---- batched DTRSV: persistent workgroups pull whole problems from a queue ----
+constexpr int kBatchMaxN = 4096;  // solution vector kept in LDS (32 KiB of doubles)
+
+__global__ __launch_bounds__(256) void trsv_batched_kernel(int uplo, int trans, int unit, int64_t batch,
+                                                           const int32_t *ns, const double *const *As,
+                                                           const int32_t *ldas, double *const *xs_g, int *ctrl) {
+  __shared__ double T[NB][NB + 1];
+  __shared__ double xsol[kBatchMaxN];
+  __shared__ double part[4][NB];
+  __shared__ int psh;
+  const int t = threadIdx.x, lane = t & 63, q = t >> 6;
+  for (;;) {
+    if (t == 0) psh = atomicAdd(&ctrl[0], 1);
+    __syncthreads();
+    const int64_t p = psh;
+    __syncthreads();
+    if (p >= batch) return;  // every wavefront of the workgroup leaves together
+    const int64_t n = ns[p];
+    if (n <= 0 || n > kBatchMaxN) continue;
+    TriView<double> v;
+    v.A = As[p];
+    v.lda = ldas[p];
+    v.n = n;
+    v.trans = trans == CORAL_CONJUGATE_TRANSPOSE ? 1 : trans;
+    const bool lower = uplo == CORAL_LOWER_TRIANGULAR;
+    v.fwd = (lower == (trans == CORAL_NO_TRANSPOSE)) ? 1 : 0;
+    double *x = xs_g[p];
+    for (int64_t i = t; i < n; i += 256) xsol[i] = x[v.map(i)];
+    __syncthreads();
+    const int nbp = (int)cdiv(n, NB);
+    double reg[16];
+    for (int b = 0; b < nbp; ++b) {
+      const int64_t i0 = (int64_t)b * NB;
+      double acc = 0.0;
+      for (int j = 0; j < b; ++j) {
+        load_tile(v, i0, (int64_t)j * NB, false, reg);
+        store_tile(v, T, reg);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          const int c = q * 16 + k;
+          acc = fma(T[c][lane], xsol[j * NB + c], acc);
+        }
+        __syncthreads();
+      }
+      part[q][lane] = acc;
+      load_tile(v, i0, i0, true, reg);
+      store_tile(v, T, reg);
+      __syncthreads();
+      if (q == 0) {
+        const int64_t i = i0 + lane;
+        const int nrows = (int)std::min<int64_t>(NB, n - i0);
+        double rhs = 0.0, dval = 0.0;
+        if (i < n) {
+          rhs = xsol[i] - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
+          if (!unit) dval = v.get(i, i);
+        }
+        double xr = solve_diag_block(rhs, T, dval, unit, nrows);
+        if (i < n) xsol[i] = xr;
+      }
+      __syncthreads();
+    }
+    for (int64_t i = t; i < n; i += 256) x[v.map(i)] = xsol[i];
+    __syncthreads();
+  }
+}
+
+}  // namespace tri
+}  // namespace coral
+
+using namespace coral;
+using namespace coral::tri;
+
+extern "C" {
+
+#define TRSV_ENTRY(NAME, E, T)                                                                               \
+  int coral_##NAME(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const T *a, int64_t lda, T *x,     \
+                   int64_t incx, coral_stream_t s) {                                                         \
+    return trsv_impl<E>(uplo, trans, diag, n, (const E *)a, lda, (E *)x, incx, s);                          \
+  }
+TRSV_ENTRY(strsv, float, float)
+TRSV_ENTRY(dtrsv, double, double)
+TRSV_ENTRY(ctrsv, float2, float)
+TRSV_ENTRY(ztrsv, double2, double)
+
+#define TRMV_ENTRY(NAME, E, T)                                                                               \
+  int coral_##NAME(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const T *a, int64_t lda, T *x,     \
+                   int64_t incx, coral_stream_t s) {                                                         \
+    return trmv_impl<E>(uplo, trans, diag, n, (const E *)a, lda, (E *)x, incx, s);                          \
+  }
+TRMV_ENTRY(strmv, float, float)
+TRMV_ENTRY(dtrmv, double, double)
+TRMV_ENTRY(ctrmv, float2, float)
+TRMV_ENTRY(ztrmv, double2, double)
+
+// uplo=1 trans=2 diag=3 batch=4 n=5 a=6 lda=7 x=8
+int coral_dtrsv_batched(int32_t uplo, int32_t trans, int32_t diag, int64_t batch, const int32_t *n,
+                        const double *const *a, const int32_t *lda, double *const *x, coral_stream_t stream) {
+  if (!valid_uplo(uplo)) return -1;
+  if (!valid_op(trans)) return -2;
+  if (!valid_diag(diag)) return -3;
+  if (batch < 0) return -4;
+  if (batch == 0) return CORAL_OK;
+  if (!n) return -5;
+  if (!a) return -6;
+  if (!lda) return -7;
+  if (!x) return -8;
+  CORAL_TRY(ensure_init());
+  hipStream_t s = resolve(stream);
+  void *ctrl = nullptr;
+  CORAL_TRY(workspace(s, WS_SMALL, 64, &ctrl));
+  hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, s, (int *)ctrl, 16);
+  CORAL_LAUNCH_CHECK();
+  const int64_t grid = std::min<int64_t>(batch, 2 * (int64_t)num_cus());
+  hipLaunchKernelGGL(trsv_batched_kernel, dim3((unsigned)grid), dim3(256), 0, s, uplo, trans,
+                     (int)(diag == CORAL_UNIT_DIAGONAL), batch, n, a, lda, x, (int *)ctrl);
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+
+}  // extern "C"

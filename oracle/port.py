@@ -1,0 +1,205 @@
+"""ctypes front end of oracle/libcoral_port.so, the CPU restatement of CORAL.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.  The
+product path (coral_amd) never imports it.
+
+Functions take the same arguments as ``coral_amd.blas`` (CORAL's argument order,
+host numpy buffers updated in place), so a test can run one call on both.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcoral_port.so")
+
+i32, i64, f32, f64, P = ctypes.c_int, ctypes.c_int64, ctypes.c_float, ctypes.c_double, ctypes.c_void_p
+
+_SIGS = {
+    "port_sgemm": (None, [i32, i32, i64, i64, i64, f32, P, i64, P, i64, f32, P, i64]),
+    "port_dgemm": (None, [i32, i32, i64, i64, i64, f64, P, i64, P, i64, f64, P, i64]),
+    "port_cgemm": (None, [i32, i32, i64, i64, i64, P, P, i64, P, i64, P, P, i64]),
+    "port_zgemm": (None, [i32, i32, i64, i64, i64, P, P, i64, P, i64, P, P, i64]),
+    "port_sgemv": (None, [i32, i64, i64, f32, P, i64, P, i64, f32, P, i64]),
+    "port_dgemv": (None, [i32, i64, i64, f64, P, i64, P, i64, f64, P, i64]),
+    "port_cgemv": (None, [i32, i64, i64, P, P, i64, P, i64, P, P, i64]),
+    "port_zgemv": (None, [i32, i64, i64, P, P, i64, P, i64, P, P, i64]),
+    "port_ssymv": (None, [i32, i64, f32, P, i64, P, i64, f32, P, i64]),
+    "port_dsymv": (None, [i32, i64, f64, P, i64, P, i64, f64, P, i64]),
+    "port_chemv": (None, [i32, i64, P, P, i64, P, i64, P, P, i64]),
+    "port_zhemv": (None, [i32, i64, P, P, i64, P, i64, P, P, i64]),
+    "port_sdot": (f32, [i64, P, i64, P, i64]),
+    "port_ddot": (f64, [i64, P, i64, P, i64]),
+    "port_snrm2": (f32, [i64, P, i64]),
+    "port_dnrm2": (f64, [i64, P, i64]),
+    "port_scnrm2": (f32, [i64, P, i64]),
+    "port_dznrm2": (f64, [i64, P, i64]),
+    "port_saxpy": (None, [i64, f32, P, i64, P, i64]),
+    "port_daxpy": (None, [i64, f64, P, i64, P, i64]),
+    "port_caxpy": (None, [i64, P, P, i64, P, i64]),
+    "port_zaxpy": (None, [i64, P, P, i64, P, i64]),
+}
+for _p in "sdcz":
+    for _w in ("trsv", "trmv"):
+        _SIGS[f"port_{_p}{_w}"] = (None, [i32, i32, i32, i64, P, i64, P, i64])
+    _SIGS[f"port_i{_p}amax"] = (i64, [i64, P, i64])
+for _n in ("cdotu", "cdotc", "zdotu", "zdotc"):
+    _SIGS[f"port_{_n}"] = (None, [i64, P, i64, P, i64, P])
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+_REAL = {"s": np.float32, "d": np.float64, "c": np.float32, "z": np.float64}
+
+
+def _ptr(a):
+    return None if a is None else np.asarray(a).ctypes.data
+
+
+def _s2(v, p):
+    real = _REAL[p]
+    arr = np.zeros(2, real)
+    v = complex(v[0], v[1]) if isinstance(v, (list, tuple, np.ndarray)) else complex(v)
+    arr[0], arr[1] = v.real, v.imag
+    return arr
+
+
+def _mk_gemm(p):
+    def f(op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+        fn = getattr(load(), f"port_{p}gemm")
+        if p in "cz":
+            al, be = _s2(alpha, p), _s2(beta, p)
+            fn(int(op_a), int(op_b), m, n, k, al.ctypes.data, _ptr(a), lda, _ptr(b), ldb, be.ctypes.data, _ptr(c), ldc)
+        else:
+            fn(int(op_a), int(op_b), m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c), ldc)
+    return f
+
+
+sgemm, dgemm, cgemm, zgemm = (_mk_gemm(p) for p in "sdcz")
+
+
+def _mk_gemv(p):
+    def f(trans, m, n, alpha, a, lda, x, incx, beta, y, incy):
+        fn = getattr(load(), f"port_{p}gemv")
+        if p in "cz":
+            al, be = _s2(alpha, p), _s2(beta, p)
+            fn(int(trans), m, n, al.ctypes.data, _ptr(a), lda, _ptr(x), incx, be.ctypes.data, _ptr(y), incy)
+        else:
+            fn(int(trans), m, n, alpha, _ptr(a), lda, _ptr(x), incx, beta, _ptr(y), incy)
+    return f
+
+
+sgemv, dgemv, cgemv, zgemv = (_mk_gemv(p) for p in "sdcz")
+
+
+def _mk_symv(p, name):
+    def f(uplo, n, alpha, a, lda, x, incx, beta, y, incy):
+        fn = getattr(load(), f"port_{name}")
+        if p in "cz":
+            al, be = _s2(alpha, p), _s2(beta, p)
+            fn(int(uplo), n, al.ctypes.data, _ptr(a), lda, _ptr(x), incx, be.ctypes.data, _ptr(y), incy)
+        else:
+            fn(int(uplo), n, alpha, _ptr(a), lda, _ptr(x), incx, beta, _ptr(y), incy)
+    return f
+
+
+ssymv, dsymv = _mk_symv("s", "ssymv"), _mk_symv("d", "dsymv")
+chemv, zhemv = _mk_symv("c", "chemv"), _mk_symv("z", "zhemv")
+
+
+def _mk_tr(p, w):
+    def f(uplo, trans, diag, n, a, lda, x, incx):
+        getattr(load(), f"port_{p}{w}")(int(uplo), int(trans), int(diag), n, _ptr(a), lda, _ptr(x), incx)
+    return f
+
+
+strsv, dtrsv, ctrsv, ztrsv = (_mk_tr(p, "trsv") for p in "sdcz")
+strmv, dtrmv, ctrmv, ztrmv = (_mk_tr(p, "trmv") for p in "sdcz")
+
+
+def sdot(n, x, incx, y, incy):
+    return load().port_sdot(n, _ptr(x), incx, _ptr(y), incy)
+
+
+def ddot(n, x, incx, y, incy):
+    return load().port_ddot(n, _ptr(x), incx, _ptr(y), incy)
+
+
+def _mk_cdot(name, real):
+    def f(n, x, incx, y, incy):
+        out = np.zeros(2, real)
+        getattr(load(), f"port_{name}")(n, _ptr(x), incx, _ptr(y), incy, out.ctypes.data)
+        return [out[0], out[1]]
+    return f
+
+
+cdotu, cdotc = _mk_cdot("cdotu", np.float32), _mk_cdot("cdotc", np.float32)
+zdotu, zdotc = _mk_cdot("zdotu", np.float64), _mk_cdot("zdotc", np.float64)
+
+
+def _mk_axpy(p):
+    def f(n, alpha, x, incx, y, incy):
+        fn = getattr(load(), f"port_{p}axpy")
+        if p in "cz":
+            fn(n, _s2(alpha, p).ctypes.data, _ptr(x), incx, _ptr(y), incy)
+        else:
+            fn(n, alpha, _ptr(x), incx, _ptr(y), incy)
+    return f
+
+
+saxpy, daxpy, caxpy, zaxpy = (_mk_axpy(p) for p in "sdcz")
+
+
+def snrm2(n, x, incx):
+    return load().port_snrm2(n, _ptr(x), incx)
+
+
+def dnrm2(n, x, incx):
+    return load().port_dnrm2(n, _ptr(x), incx)
+
+
+def scnrm2(n, x, incx):
+    return load().port_scnrm2(n, _ptr(x), incx)
+
+
+def dznrm2(n, x, incx):
+    return load().port_dznrm2(n, _ptr(x), incx)
+
+
+def isamax(n, x, incx):
+    return int(load().port_isamax(n, _ptr(x), incx))
+
+
+def idamax(n, x, incx):
+    return int(load().port_idamax(n, _ptr(x), incx))
+
+
+def icamax(n, x, incx):
+    return int(load().port_icamax(n, _ptr(x), incx))
+
+
+def izamax(n, x, incx):
+    return int(load().port_izamax(n, _ptr(x), incx))

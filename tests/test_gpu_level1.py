@@ -1,0 +1,116 @@
+"""GPU parity for dot / axpy / nrm2 / i?amax / scal through the C ABI.
+
+i?amax is bit-exact (0-based first index, NaN skipped, |re|+|im|) against brute
+force and the oracle; the float reductions are checked against correctly rounded
+FP64 references with the reference tests' tolerances (dot_tests.rs:176-179).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from coral_amd import blas
+from oracle import port, ref64
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_dot(gpu_lib):
+    rng = np.random.default_rng(30)
+    for n, inc in ((0, 1), (1, 1), (31, 1), (1000, 2), (100003, 1), (1 << 20, 1)):
+        x = rng.uniform(-1, 1, (n - 1) * inc + 1 if n else 1)
+        y = rng.uniform(-1, 1, (n - 1) * inc + 1 if n else 1)
+        e = ref64.dot_exact(ref64.vec(x, n, inc), ref64.vec(y, n, inc))
+        assert abs(blas.ddot(n, x, inc, y, inc) - e) <= 1e-12 * max(1, np.sqrt(n))
+        xf, yf = x.astype(np.float32), y.astype(np.float32)
+        ef = ref64.dot_exact(ref64.vec(xf, n, inc), ref64.vec(yf, n, inc))
+        assert abs(blas.sdot(n, xf, inc, yf, inc) - ef) <= 4e-6 * max(1, np.sqrt(n))
+        xc = (x + 1j * y).astype(np.complex128)
+        yc = (y - 0.5j * x).astype(np.complex128)
+        for name, conj in (("zdotu", False), ("zdotc", True)):
+            r = getattr(blas, name)(n, xc, inc, yc, inc)
+            e = ref64.dot_exact(ref64.vec(xc, n, inc), ref64.vec(yc, n, inc), conj=conj)
+            assert abs(complex(*r) - e) <= 1e-12 * max(1, np.sqrt(n)), name
+        for name, conj in (("cdotu", False), ("cdotc", True)):
+            r = getattr(blas, name)(n, xc.astype(np.complex64), inc, yc.astype(np.complex64), inc)
+            e = ref64.dot_exact(ref64.vec(xc.astype(np.complex64), n, inc), ref64.vec(yc.astype(np.complex64), n, inc),
+                                conj=conj)
+            assert abs(complex(*r) - e) <= 4e-6 * max(1, np.sqrt(n)), name
+
+
+def test_axpy(gpu_lib):
+    rng = np.random.default_rng(31)
+    for n, incx, incy in ((0, 1, 1), (1, 1, 1), (1000, 1, 1), (777, 3, 2)):
+        x = rng.uniform(-1, 1, (n - 1) * incx + 1 if n else 1)
+        y = rng.uniform(-1, 1, (n - 1) * incy + 1 if n else 1)
+        out = y.copy()
+        blas.daxpy(n, 0.5, x, incx, out, incy)
+        po = y.copy()
+        port.daxpy(n, 0.5, x, incx, po, incy)
+        assert np.array_equal(out, po)  # both one fma per element
+        out = y.copy()
+        blas.daxpy(n, 0.0, np.full_like(x, np.nan), incx, out, incy)  # alpha == 0 quick return
+        assert np.array_equal(out, y)
+        xc = (x + 0.5j * x).astype(np.complex64)
+        yc = (y - 1j * y).astype(np.complex64)
+        out = yc.copy()
+        blas.caxpy(n, (0.5, -2.0), xc, incx, out, incy)
+        ref = yc.astype(np.complex128).copy()
+        ref[: (n - 1) * incy + 1: incy] += (0.5 - 2.0j) * xc[: (n - 1) * incx + 1: incx] if n else 0
+        assert np.abs(out - ref).max() <= 1e-6 if n else True
+
+
+def test_nrm2(gpu_lib):
+    rng = np.random.default_rng(32)
+    for scale in (1e-30, 1e-10, 1.0, 1e10, 1e30):
+        x = (rng.uniform(-1, 1, 10001) * scale).astype(np.float32)
+        e = ref64.nrm2(x)
+        assert abs(blas.snrm2(10001, x, 1) - e) <= 1e-6 * e
+    for scale in (1e-300, 1e-160, 1.0, 1e160, 1e300):
+        x = rng.uniform(-1, 1, 10001) * scale
+        e = ref64.nrm2(x)
+        assert abs(blas.dnrm2(10001, x, 1) - e) <= 1e-14 * e
+        xc = (x + 1j * x[::-1]).astype(np.complex128)
+        e = ref64.nrm2(xc)
+        assert abs(blas.dznrm2(10001, xc, 1) - e) <= 1e-14 * e
+    x = rng.uniform(-1, 1, 3001)
+    assert abs(blas.dnrm2(1000, x, 3) - ref64.nrm2(x[::3])) <= 1e-14 * ref64.nrm2(x[::3])
+    assert blas.dnrm2(0, x, 1) == 0.0
+    assert np.isnan(blas.dnrm2(3, np.array([1.0, np.nan, 2.0]), 1))
+    assert np.isinf(blas.dnrm2(3, np.array([1.0, np.inf, 2.0]), 1))
+
+
+def test_iamax_bit_exact(gpu_lib):
+    z = np.load(os.path.join(GOLD, "iamax_cases.npz"))
+    assert blas.isamax(1024, z["x1"], 1) == int(z["expect1"])
+    assert blas.isamax(int(z["n2"]), z["x2"], int(z["inc2"])) == int(z["expect2"])
+    assert blas.isamax(0, np.ones(5, np.float32), 1) == 0
+    x = np.array([1.0, -3.0, np.nan, 3.0, 2.0])
+    assert blas.idamax(5, x, 1) == 1
+    assert blas.idamax(3, np.full(3, np.nan), 1) == 0
+    xc = np.array([1 + 2j, -2 - 1j, 3 + 0j, 0 - 3j], np.complex64)
+    assert blas.icamax(4, xc, 1) == 0
+    rng = np.random.default_rng(33)
+    for n in (1, 2, 63, 64, 65, 1000, 100000, 1 << 20):
+        v = rng.integers(-50, 51, n).astype(np.float64)  # many ties
+        v[rng.integers(0, n, max(1, n // 100))] = np.nan
+        assert blas.idamax(n, v, 1) == port.idamax(n, v, 1) == ref64.iamax(v, n, 1), n
+        vf = v.astype(np.float32)
+        assert blas.isamax(n, vf, 1) == port.isamax(n, vf, 1), n
+        vc = (v + 1j * v[::-1]).astype(np.complex128)
+        assert blas.izamax(n, vc, 1) == port.izamax(n, vc, 1), n
+        assert blas.icamax(n, vc.astype(np.complex64), 1) == port.icamax(n, vc.astype(np.complex64), 1), n
+    v = rng.uniform(-1, 1, 3000)
+    assert blas.idamax(1000, v, 3) == port.idamax(1000, v, 3)
+
+
+def test_scal(gpu_lib):
+    from coral_amd.device import DeviceArray
+    from coral_amd._lib import call
+
+    x = np.array([1.0, np.nan, -2.0, 4.0])
+    d = DeviceArray.from_numpy(x)
+    call("coral_dscal", 4, 0.0, d.ptr, 1, None)  # multiplies: NaN stays NaN (dscal.rs:30-66)
+    r = d.to_numpy()
+    assert r[0] == 0.0 and np.isnan(r[1]) and r[2] == 0.0

@@ -1,0 +1,153 @@
+"""GPU parity for GEMV / SYMV / HEMV / TRSV / TRMV through the C ABI.
+
+Compared with the oracle (CORAL restatement) and the FP64 reference with the
+reference tests' tolerances (dgemv_tests.rs:117-118, sgemv_tests.rs:116-117,
+chemv_tests.rs:190-191, dtrsv_tests.rs:213-214), plus the backward error
+|b - A x| / (|A| |x| + |b|) for TRSV.
+"""
+import numpy as np
+import pytest
+
+from coral_amd import blas
+from oracle import port, ref64
+from tests.conftest import allclose_ref
+
+pytestmark = pytest.mark.gpu
+DT = {"s": np.float32, "d": np.float64, "c": np.complex64, "z": np.complex128}
+TOL = {"s": (1e-5, 1e-5), "d": (1e-13, 1e-13), "c": (1e-5, 1e-6), "z": (1e-12, 1e-12)}
+
+
+def rand(rng, n, f):
+    if np.issubdtype(f, np.complexfloating):
+        return (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).astype(f)
+    return rng.uniform(-1, 1, n).astype(f)
+
+
+@pytest.mark.parametrize("p", "sdcz")
+def test_gemv_all_variants(gpu_lib, p):
+    rng = np.random.default_rng(20)
+    f = DT[p]
+    cplx = p in "cz"
+    fn = getattr(blas, f"{p}gemv")
+    pfn = getattr(port, f"{p}gemv")
+    for (m, n, lda, incx, incy) in ((1, 1, 1, 1, 1), (7, 5, 9, 1, 1), (1024, 512, 1024, 1, 1),
+                                    (1000, 3000, 1003, 2, 3), (4097, 33, 4100, 3, 2), (33, 4097, 33, 1, 1)):
+        a = rand(rng, lda * n, f)
+        for trans in (0, 1, 2):
+            xl, yl = (n, m) if trans == 0 else (m, n)
+            x = rand(rng, (xl - 1) * incx + 1, f)
+            y = rand(rng, (yl - 1) * incy + 1, f)
+            al, be = ((0.75, -0.25), (0.85, 0.1)) if cplx else (0.75, -0.25)
+            out = y.copy()
+            fn(trans, m, n, al, a, lda, x, incx, be, out, incy)
+            ref = ref64.gemv(trans, m, n, al, a, lda, x, incx, be, y, incy)
+            scale = np.abs(ref).max() + 1e-30
+            rtol = 1e-5 if p in "sc" else 1e-13
+            err = np.abs(out.astype(ref.dtype) - ref).max() / scale
+            assert err <= rtol * max(1.0, np.sqrt(xl) / 8), (m, n, trans, err)
+            po = y.copy()
+            pfn(trans, m, n, al, a, lda, x, incx, be, po, incy)
+            assert np.abs(out - po).max() / scale <= rtol * max(1.0, np.sqrt(xl) / 8)
+
+
+def test_gemv_quick_returns_and_beta_zero(gpu_lib):
+    y = np.array([1.0, np.nan, 3.0])
+    out = y.copy()
+    blas.dgemv(0, 3, 2, 0.0, np.zeros(6), 3, np.zeros(2), 1, 1.0, out, 1)
+    assert out[0] == 1.0 and np.isnan(out[1]) and out[2] == 3.0
+    # beta == 0 overwrites y (BLAS semantics; CORAL's dscal pre-scaling keeps NaN)
+    out = y.copy()
+    blas.dgemv(0, 3, 2, 1.0, np.ones(6), 3, np.ones(2), 1, 0.0, out, 1)
+    assert np.array_equal(out, [2.0, 2.0, 2.0])
+    out = y.copy()
+    blas.dgemv(1, 3, 2, 0.0, np.ones(6), 3, np.ones(3), 1, 2.0, out, 1)  # alpha 0: y := beta*y
+    assert out[0] == 2.0 and out[2] == 6.0
+
+
+@pytest.mark.parametrize("p", "sdcz")
+@pytest.mark.parametrize("uplo", [0, 1])
+def test_symv_hemv(gpu_lib, p, uplo):
+    herm = p in "cz"
+    f = DT[p]
+    fn = {"s": blas.ssymv, "d": blas.dsymv, "c": blas.chemv, "z": blas.zhemv}[p]
+    rng = np.random.default_rng(21)
+    for n, lda, incx, incy in ((1, 1, 1, 1), (37, 40, 1, 1), (128, 128, 2, 1), (300, 301, 1, 3), (1000, 1000, 1, 1)):
+        a = rand(rng, lda * n, f)
+        A = a.reshape(n, lda).T.copy()
+        for j in range(n):  # NaN in the unreferenced triangle
+            lo, hi = (j + 1, n) if uplo == 0 else (0, j)
+            A[lo:hi, j] = np.nan
+        a = A.T.reshape(-1).copy()
+        x = rand(rng, (n - 1) * incx + 1, f)
+        y = rand(rng, (n - 1) * incy + 1, f)
+        al, be = ((1.25, -0.5), (0.5, 0.25)) if herm else (1.25, 0.5)
+        out = y.copy()
+        fn(uplo, n, al, a, lda, x, incx, be, out, incy)
+        ref = ref64.symv(uplo, n, al, a, lda, x, incx, be, y, incy, herm=herm)
+        assert np.all(np.isfinite(out))
+        rtol = 2e-5 if p in "sc" else 1e-12
+        assert np.abs(out - ref).max() <= rtol * (np.abs(ref).max() + 1) * max(1, np.sqrt(n) / 8), (n,)
+    # upper and lower storage of the same matrix agree
+    n = 65
+    M = rand(rng, n * n, f).reshape(n, n)
+    H = (M + M.conj().T) / 2 if herm else (M + M.T) / 2
+    h = np.asfortranarray(H).reshape(-1, order="F").astype(f)
+    x = rand(rng, n, f)
+    o_u, o_l = np.zeros(n, f), np.zeros(n, f)
+    one, zero = ((1, 0), (0, 0)) if herm else (1.0, 0.0)
+    fn(0, n, one, h, n, x, 1, zero, o_u, 1)
+    fn(1, n, one, h, n, x, 1, zero, o_l, 1)
+    assert np.abs(o_u - o_l).max() <= (1e-5 if p in "sc" else 1e-13) * max(1, np.abs(o_u).max())
+
+
+def tri(n, lda, f, rng, uplo, cplx):
+    a = np.full(lda * n, np.nan, f)
+    for j in range(n):
+        rows = range(j, n) if uplo == 1 else range(0, j + 1)
+        for i in rows:
+            v = rng.uniform(-1, 1) / n + (1j * rng.uniform(-1, 1) / n if cplx else 0)
+            a[i + j * lda] = v if i != j else rng.uniform(1, 2) + (0.25j if cplx else 0)
+    return a
+
+
+@pytest.mark.parametrize("p", "sdcz")
+def test_trsv_trmv_all_variants(gpu_lib, p):
+    cplx = p in "cz"
+    f = DT[p]
+    rng = np.random.default_rng(22)
+    eps = 2.0 ** -24 if p in "sc" else 2.0 ** -53
+    for n, lda, incx in ((1, 1, 1), (7, 7, 1), (64, 64, 2), (65, 70, 1), (200, 200, 3), (640, 640, 1)):
+        for uplo in (0, 1):
+            a = tri(n, lda, f, rng, uplo, cplx)
+            for trans in (0, 1, 2):
+                for diag in (0, 1):
+                    a_use = a.copy()
+                    if diag == 0:
+                        for j in range(n):
+                            a_use[j + j * lda] = np.nan  # unit diagonal is never read
+                    x = rand(rng, (n - 1) * incx + 1, f)
+                    xs = x.copy()
+                    getattr(blas, f"{p}trsv")(uplo, trans, diag, n, a_use, lda, xs, incx)
+                    M = ref64.triangular(a_use, n, lda, uplo, trans, diag)
+                    xv = ref64.vec(xs, n, incx).astype(M.dtype)
+                    bv = ref64.vec(x, n, incx).astype(M.dtype)
+                    berr = np.linalg.norm(bv - M @ xv) / (np.linalg.norm(M) * np.linalg.norm(xv) + np.linalg.norm(bv))
+                    assert berr <= 4 * n * eps, (n, uplo, trans, diag, berr)
+                    ref = ref64.trsv(uplo, trans, diag, n, a_use, lda, x, incx)
+                    assert allclose_ref(xs, ref, 1e-4 if p in "sc" else 1e-12, 1e-5 if p in "sc" else 1e-12)
+                    # TRMV of the solution returns the right-hand side
+                    getattr(blas, f"{p}trmv")(uplo, trans, diag, n, a_use, lda, xs, incx)
+                    assert allclose_ref(xs, x, 1e-4 if p in "sc" else 1e-12, 1e-5 if p in "sc" else 1e-12)
+
+
+def test_dtrsv_n8192_backward_error(gpu_lib):
+    # the DTRSV configuration: lower, non-unit, n = 8192, strictly-lower U[-1,1]/n, diag U[1,2]
+    rng = np.random.default_rng(23)
+    n = 8192
+    A = np.tril(rng.uniform(-1, 1, (n, n)) / n, -1) + np.diag(rng.uniform(1, 2, n))
+    a = np.asfortranarray(A).reshape(-1, order="F")
+    b = rng.uniform(-1, 1, n)
+    x = b.copy()
+    blas.dtrsv(1, 0, 1, n, a, n, x, 1)
+    berr = np.linalg.norm(b - A @ x) / (np.linalg.norm(A) * np.linalg.norm(x) + np.linalg.norm(b))
+    assert berr <= 2.0 ** -53 * 16, berr

@@ -1,0 +1,194 @@
+"""GPU parity for the level-3 routines, through the C ABI (libcoralhip.so).
+
+Checked against the oracle (CORAL restatement, oracle/coral_port.c) with the
+reference tests' tolerances, and against FP64 with the north-star normwise bound
+|C - C_ref|_F / (k * eps * |A|_F * |B|_F) <= 1.  The split-precision forms carry
+the BF16 hi+lo representation error (about 2^-17 per product), so their bound
+uses eps_split = 2^-16 + k * eps32 in place of k * eps.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from coral_amd import blas
+from oracle import port, ref64
+from tests.conftest import allclose_ref
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+GEMM_TOL = {"s": (1e-3, 1e-3), "d": (1e-12, 1e-12), "c": (3e-3, 2e-3), "z": (3e-11, 2e-11)}
+EPS = {"s": 2.0 ** -24, "d": 2.0 ** -53, "c": 2.0 ** -24, "z": 2.0 ** -53}
+GPU_FN = {"s": blas.sgemm, "d": blas.dgemm, "c": blas.cgemm, "z": blas.zgemm}
+PORT_FN = {"s": port.sgemm, "d": port.dgemm, "c": port.cgemm, "z": port.zgemm}
+DT = {"s": np.float32, "d": np.float64, "c": np.complex64, "z": np.complex128}
+
+
+def normwise(out, ref, a_fro, b_fro, k, eps):
+    return np.linalg.norm(np.asarray(out).astype(ref.dtype) - ref) / (max(k, 1) * eps * a_fro * b_fro + 1e-300)
+
+
+def rand(rng, n, f):
+    if np.issubdtype(f, np.complexfloating):
+        return (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).astype(f)
+    return rng.uniform(-1, 1, n).astype(f)
+
+
+def test_kat(gpu_lib):
+    kat = json.load(open(os.path.join(GOLD, "kat.json")))
+    k = kat["sgemm_2x2"]
+    for fn, f in ((blas.sgemm, np.float32), (blas.dgemm, np.float64), (blas.sgemm_bf16x3, np.float32)):
+        c = np.array(k["c"], f)
+        fn(0, 0, 2, 2, 2, k["alpha"], np.array(k["a"], f), 2, np.array(k["b"], f), 2, k["beta"], c, 2)
+        assert np.all(np.abs(c - np.array(k["expect"])) < 1e-6), fn.__name__
+    k = kat["cgemm_2x2"]
+    for fn, f in ((blas.cgemm, np.float32), (blas.zgemm, np.float64), (blas.cgemm_3m_bf16x3, np.float32)):
+        c = np.array(k["c"], f)
+        fn(0, 0, 2, 2, 2, k["alpha"], np.array(k["a"], f), 2, np.array(k["b"], f), 2, k["beta"], c, 2)
+        assert np.all(np.abs(c - np.array(k["expect"])) < 1e-6), fn.__name__
+
+
+def test_golden_reference_generators(gpu_lib):
+    z = np.load(os.path.join(GOLD, "gemm_cases.npz"))
+    index = json.loads(str(z["index"]))
+    for key, dt, m, n, k, oa, ob, lda, ldb, ldc, alpha, beta in index:
+        a, b, c, expect = z[key + "_a"], z[key + "_b"], z[key + "_c"], z[key + "_expect"]
+        out = c.copy()
+        GPU_FN[dt](oa, ob, m, n, k, alpha, a, lda, b, ldb, beta, out, ldc)
+        assert allclose_ref(out, expect, *GEMM_TOL[dt]), key
+        # ld padding between columns is untouched
+        for j in range(n):
+            assert np.array_equal(out[j * ldc + m:(j + 1) * ldc], c[j * ldc + m:(j + 1) * ldc]), key
+        po = c.copy()
+        PORT_FN[dt](oa, ob, m, n, k, alpha, a, lda, b, ldb, beta, po, ldc)
+        assert allclose_ref(out, po, *GEMM_TOL[dt]), key
+
+
+SHAPES = [(1, 1, 1), (5, 7, 4), (64, 64, 64), (127, 129, 65), (300, 257, 129), (17, 513, 40), (512, 16, 777)]
+
+
+@pytest.mark.parametrize("dt", ["s", "d", "c", "z"])
+def test_normwise_bound_all_ops(gpu_lib, dt):
+    rng = np.random.default_rng(10)
+    f = DT[dt]
+    ops = [0, 1, 2]
+    worst = 0.0
+    for (m, n, k) in SHAPES:
+        for oa in ops:
+            for ob in ops:
+                lda = (m if oa == 0 else k) + 3
+                ldb = (k if ob == 0 else n) + 1
+                ldc = m + 2
+                ar, ac = (m, k) if oa == 0 else (k, m)
+                br, bc = (k, n) if ob == 0 else (n, k)
+                a = rand(rng, lda * ac, f)
+                b = rand(rng, ldb * bc, f)
+                c = rand(rng, ldc * n, f)
+                cplx = dt in "cz"
+                al, be = ((1.25, -0.5), (0.5, 0.25)) if cplx else (1.25, 0.5)
+                out = c.copy()
+                GPU_FN[dt](oa, ob, m, n, k, al, a, lda, b, ldb, be, out, ldc)
+                ref = ref64.gemm(oa, ob, m, n, k, al, a, lda, b, ldb, be, c, ldc)
+                A = ref64.mat(a, ar, ac, lda)
+                B = ref64.mat(b, br, bc, ldb)
+                C = ref64.mat(c, m, n, ldc)
+                scale = abs(complex(*al)) if cplx else abs(al)
+                num = np.linalg.norm(ref64.mat(out, m, n, ldc).astype(ref.dtype) - ref64.mat(ref, m, n, ldc))
+                den = max(k, 1) * EPS[dt] * (scale * np.linalg.norm(A) * np.linalg.norm(B) + np.linalg.norm(C))
+                worst = max(worst, num / den)
+    assert worst <= 1.0, worst
+
+
+@pytest.mark.parametrize("which", ["sgemm_bf16x3", "cgemm_3m_bf16x3"])
+def test_split_precision_bound(gpu_lib, which):
+    rng = np.random.default_rng(11)
+    cplx = which.startswith("c")
+    f = np.complex64 if cplx else np.float32
+    fn = getattr(blas, which)
+    ops = [0, 1, 2] if cplx else [0, 1]
+    worst = 0.0
+    for (m, n, k) in SHAPES:
+        for oa in ops:
+            for ob in ops:
+                lda, ldb, ldc = (m if oa == 0 else k), (k if ob == 0 else n), m
+                ar, ac = (m, k) if oa == 0 else (k, m)
+                br, bc = (k, n) if ob == 0 else (n, k)
+                a, b, c = rand(rng, lda * ac, f), rand(rng, ldb * bc, f), rand(rng, ldc * n, f)
+                al, be = ((1.0, 0.5), (0.5, -0.25)) if cplx else (1.0, 0.5)
+                out = c.copy()
+                fn(oa, ob, m, n, k, al, a, lda, b, ldb, be, out, ldc)
+                ref = ref64.gemm(oa, ob, m, n, k, al, a, lda, b, ldb, be, c, ldc)
+                A, B = ref64.mat(a, ar, ac, lda), ref64.mat(b, br, bc, ldb)
+                eps_split = 2.0 ** -16 + k * 2.0 ** -24
+                # 3M adds the Gauss cancellation: Im = T3 - T1 - T2 (three products' errors)
+                mult = 3.0 if cplx else 1.0
+                num = np.linalg.norm(out.astype(ref.dtype) - ref)
+                den = mult * eps_split * (abs(complex(*al)) if cplx else al) * np.linalg.norm(A) * np.linalg.norm(B)
+                worst = max(worst, num / (den + np.linalg.norm(c) * 2.0 ** -24))
+    assert worst <= 1.0, worst
+
+
+def test_split_precision_wide_range_rows(gpu_lib):
+    # rows of A and columns of B scaled by 2^e, e in [-20, 20]: the power-of-two
+    # row/column scaling keeps the relative error of every element at split level
+    rng = np.random.default_rng(12)
+    m = n = k = 256
+    a = rng.standard_normal((m, k)).astype(np.float32)
+    b = rng.standard_normal((k, n)).astype(np.float32)
+    ea = rng.integers(-20, 21, m)
+    eb = rng.integers(-20, 21, n)
+    a = (a * np.ldexp(1.0, ea)[:, None]).astype(np.float32)
+    b = (b * np.ldexp(1.0, eb)[None, :]).astype(np.float32)
+    af, bf = np.asfortranarray(a).reshape(-1, order="F"), np.asfortranarray(b).reshape(-1, order="F")
+    out = np.zeros(m * n, np.float32)
+    blas.sgemm_bf16x3(0, 0, m, n, k, 1.0, af, m, bf, k, 0.0, out, m)
+    ref = a.astype(np.float64) @ b.astype(np.float64)
+    scale = np.ldexp(1.0, ea)[:, None] * np.ldexp(1.0, eb)[None, :]
+    rel = np.abs(out.reshape(n, m).T - ref) / (scale * np.sqrt(k))
+    assert rel.max() < 2.0 ** -14, rel.max()
+
+
+@pytest.mark.parametrize("dt", ["s", "d", "c", "z"])
+def test_beta_zero_never_reads_c_and_quick_returns(gpu_lib, dt):
+    rng = np.random.default_rng(13)
+    f = DT[dt]
+    cplx = dt in "cz"
+    m, n, k = 33, 47, 21
+    a, b = rand(rng, m * k, f), rand(rng, k * n, f)
+    one, zero, half = ((1.0, 0.0), (0.0, 0.0), (0.5, 0.0)) if cplx else (1.0, 0.0, 0.5)
+    c = np.full(m * n, np.nan, f)
+    GPU_FN[dt](0, 0, m, n, k, one, a, m, b, k, zero, c, m)
+    assert np.all(np.isfinite(c))
+    # alpha == 0: C := beta*C with A and B unread (NaN operands)
+    c0 = rand(rng, m * n, f)
+    c1 = c0.copy()
+    GPU_FN[dt](0, 0, m, n, k, zero, np.full(m * k, np.nan, f), m, np.full(k * n, np.nan, f), k, half, c1, m)
+    assert np.allclose(c1, c0 * 0.5, rtol=0, atol=0)
+    # k == 0 with beta == 0 zeroes C
+    c2 = np.full(m * n, np.nan, f)
+    GPU_FN[dt](0, 0, m, n, 0, one, a, m, b, 1, zero, c2, m)
+    assert np.all(c2 == 0)
+
+
+def test_real_conjtranspose_equals_transpose(gpu_lib):
+    rng = np.random.default_rng(14)
+    m, n, k = 40, 30, 50
+    a = rng.uniform(-1, 1, k * m).astype(np.float32)
+    b = rng.uniform(-1, 1, k * n).astype(np.float32)
+    c1, c2 = np.zeros(m * n, np.float32), np.zeros(m * n, np.float32)
+    blas.sgemm(1, 0, m, n, k, 1.0, a, k, b, k, 0.0, c1, m)
+    blas.sgemm(2, 0, m, n, k, 1.0, a, k, b, k, 0.0, c2, m)
+    assert np.array_equal(c1, c2)
+
+
+def test_argument_errors_reported(gpu_lib):
+    from coral_amd import CoralError
+
+    a = np.zeros(16, np.float32)
+    with pytest.raises(CoralError) as e:
+        blas.sgemm(0, 0, 4, 4, 4, 1.0, a, 3, a, 4, 0.0, a, 4)  # lda < m
+    assert e.value.status == -8
+    with pytest.raises(CoralError) as e:
+        blas.sgemm(7, 0, 4, 4, 4, 1.0, a, 4, a, 4, 0.0, a, 4)
+    assert e.value.status == -1
