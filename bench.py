@@ -107,6 +107,315 @@ def cpu_baseline(n=3072, runs=3):
                       f"({os.cpu_count()} visible), {t:.2f} s per run"}
 
 
+# ---------------------------------------------------------------------------
+# --suite: every configuration of the benchmark plan, one JSON line each
+# ---------------------------------------------------------------------------
+def _emit(d):
+    print(json.dumps(d), flush=True)
+
+
+def _fro_sample_err(A, B, C, rows, k, alpha=1.0, eps=2.0 ** -24, cplx=False, ops=(0, 0)):
+    """Row-sample estimate of |C - C_ref|_F / (k eps |A|_F |B|_F) and max |E| / max |C_ref|."""
+    wide = np.complex128 if cplx else np.float64
+    opA = A if ops[0] == 0 else (A.T if ops[0] == 1 else A.conj().T)
+    opB = B if ops[1] == 0 else (B.T if ops[1] == 1 else B.conj().T)
+    ref = alpha * (opA[rows, :].astype(wide) @ opB.astype(wide))
+    E = C[rows, :].astype(wide) - ref
+    scale = np.sqrt(C.shape[0] / len(rows))
+    fa, fb = np.linalg.norm(opA.astype(wide)), np.linalg.norm(opB.astype(wide))
+    return {"normwise_bound_ratio": float(scale * np.linalg.norm(E) / (k * eps * abs(alpha) * fa * fb)),
+            "fro_rel_to_AB": float(scale * np.linalg.norm(E) / (abs(alpha) * fa * fb)),
+            "max_abs_over_max_ref": float(np.abs(E).max() / np.abs(ref).max())}
+
+
+def _host_mat(d, rows, cols, dtype):
+    return d.to_numpy().view(dtype).reshape(cols, rows).T
+
+
+def suite_c1():
+    from coral_amd import device
+    from coral_amd._lib import call
+    from oracle import port
+
+    n = 512
+    rng = np.random.default_rng(100)
+    a, b, c = (rng.uniform(-1, 1, n * n).astype(np.float32) for _ in range(3))
+    A, B, C = (device.DeviceArray.from_numpy(x) for x in (a, b, c))
+    ms = timed_steps(lambda: call("coral_sgemm", 0, 0, n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.5, C.ptr, n, None), 50, 5)
+    C.upload(c)
+    call("coral_sgemm", 0, 0, n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.5, C.ptr, n, None)
+    out = C.to_numpy()
+    po = c.copy()
+    t0 = time.perf_counter()
+    port.sgemm(0, 0, n, n, n, 1.0, a, n, b, n, 0.5, po, n)  # cpu_baseline leg: CORAL's algorithm restated
+    cpu_s = time.perf_counter() - t0
+    ref = (a.reshape(n, n).T.astype(np.float64) @ b.reshape(n, n).T.astype(np.float64)) + 0.5 * c.reshape(n, n).T
+    err = np.linalg.norm(out.reshape(n, n).T - ref) / (n * 2.0 ** -24 * np.linalg.norm(a) * np.linalg.norm(b))
+    _emit({"config": "C1 SGEMM NN 512^3 alpha=1 beta=0.5 U[-1,1]", "ms": round(ms, 4),
+           "tflops": round(2 * n ** 3 / ms / 1e9, 2), "normwise_bound_ratio": float(err),
+           "max_abs_diff_vs_oracle": float(np.abs(out - po).max()),
+           "cpu_baseline_port_gflops_1core": round(2 * n ** 3 / cpu_s / 1e9, 2)})
+
+
+def suite_c2():
+    from coral_amd import device, synthetic
+    from coral_amd._lib import call
+
+    n = 8192
+    A, B, C = (device.DeviceArray(n * n, np.float64) for _ in range(3))
+    device.fill_random(A, synthetic.seed_for(2, "A"), "normal")
+    device.fill_random(B, synthetic.seed_for(2, "B"), "normal")
+    for ops in ((0, 0), (1, 1)):
+        call("coral_timing_reset")
+        call("coral_timing_enable", 1)
+        ms = timed_steps(lambda: call("coral_dgemm", ops[0], ops[1], n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.0, C.ptr, n,
+                                      None), 3, 1)
+        k_ms, _ = kernel_avg_ms(0)
+        call("coral_timing_enable", 0)
+        Ah, Bh, Ch = (_host_mat(x, n, n, np.float64) for x in (A, B, C))
+        rows = np.random.default_rng(2).choice(n, 64, replace=False)
+        err = _fro_sample_err(Ah, Bh, Ch, rows, n, eps=2.0 ** -53, ops=ops)
+        tf = 2 * n ** 3 / ms / 1e9
+        _emit({"config": f"C2 DGEMM {'NN' if ops == (0, 0) else 'TT'} 8192^3 N(0,1)", "ms": round(ms, 3),
+               "tflops": round(tf, 2), "frac_fp64_peak": round(tf / 78.6, 4),
+               "kernel_tflops": round(2 * n ** 3 / k_ms / 1e9, 2), "error_vs_fp64": err})
+        del Ah, Bh, Ch
+    del A, B, C
+    # DGEMV N 16384^2
+    m = 16384
+    GA, gx, gy = device.DeviceArray(m * m, np.float64), device.DeviceArray(m, np.float64), \
+        device.DeviceArray(m, np.float64)
+    for d, o in ((GA, "A"), (gx, "x"), (gy, "y")):
+        device.fill_random(d, synthetic.seed_for(2, o), "uniform")
+    for tr in (0, 1):
+        call("coral_timing_reset")
+        call("coral_timing_enable", 1)
+        ms = timed_steps(lambda: call("coral_dgemv", tr, m, m, 1.0, GA.ptr, m, gx.ptr, 1, 0.0, gy.ptr, 1, None), 20, 2)
+        k_ms, _ = kernel_avg_ms(1)
+        call("coral_timing_enable", 0)
+        by = 8.0 * (m * m + 2 * m)
+        _emit({"config": f"C2 DGEMV {'N' if tr == 0 else 'T'} 16384^2 U[-1,1] beta=0", "us": round(ms * 1e3, 1),
+               "gbps": round(by / ms / 1e6, 1), "kernel_gbps": round(8.0 * (m * m + m) / k_ms / 1e6, 1),
+               "frac_hbm": round(by / ms / 1e6 / PEAK_HBM, 4)})
+    del GA, gx, gy
+    # DTRSV lower non-unit 8192
+    n = 8192
+    rng = np.random.default_rng(3)
+    Ah = np.tril(rng.uniform(-1, 1, (n, n)) / n, -1) + np.diag(rng.uniform(1, 2, n))
+    b = rng.uniform(-1, 1, n)
+    A = device.DeviceArray.from_numpy(np.asfortranarray(Ah).reshape(-1, order="F"))
+    X = device.DeviceArray.from_numpy(b)
+    ms = timed_steps(lambda: (X.upload(b), call("coral_dtrsv", 1, 0, 1, n, A.ptr, n, X.ptr, 1, None)), 1, 1)
+    e0, e1 = device.Event(), device.Event()
+    times = []
+    for _ in range(10):
+        X.upload(b)
+        device.synchronize()
+        e0.record()
+        call("coral_dtrsv", 1, 0, 1, n, A.ptr, n, X.ptr, 1, None)
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_ms(e1))
+    ms = statistics.median(times)
+    x = X.to_numpy()
+    berr = np.linalg.norm(b - Ah @ x) / (np.linalg.norm(Ah) * np.linalg.norm(x) + np.linalg.norm(b))
+    by = 8.0 * (n * (n + 1) / 2 + 2 * n)
+    _emit({"config": "C2 DTRSV lower non-unit 8192", "us": round(ms * 1e3, 1), "gbps": round(by / ms / 1e6, 1),
+           "gflops": round(n * n / ms / 1e6, 2), "backward_error": float(berr)})
+
+
+def suite_c3():
+    from coral_amd import device, synthetic
+    from coral_amd._lib import call
+
+    for n in (4096, 8192, 16384):
+        A, B, C, C2 = (device.DeviceArray(n * n, np.float32) for _ in range(4))
+        for dist in ("normal", "wide"):
+            device.fill_random(A, synthetic.seed_for(3, "A") + n, "normal")
+            device.fill_random(B, synthetic.seed_for(3, "B") + n, "normal")
+            if dist == "wide":
+                rng = np.random.default_rng(n)
+                ea = device.DeviceArray.from_numpy(rng.integers(-20, 21, n).astype(np.int32))
+                eb = device.DeviceArray.from_numpy(rng.integers(-20, 21, n).astype(np.int32))
+                call("coral_scale_pow2_f32", A.ptr, n, n, n, ea.ptr, None, None)
+                call("coral_scale_pow2_f32", B.ptr, n, n, n, None, eb.ptr, None)
+            steps = 3 if n >= 16384 else 5
+            ms3 = timed_steps(lambda: call("coral_sgemm_bf16x3", 0, 0, n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.0, C.ptr,
+                                           n, None), steps, 1)
+            msp = timed_steps(lambda: call("coral_sgemm", 0, 0, n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.0, C2.ptr, n,
+                                           None), steps, 1)
+            Ah, Bh, Cb, Cp = (_host_mat(x, n, n, np.float32) for x in (A, B, C, C2))
+            rows = np.random.default_rng(5).choice(n, 128, replace=False)
+            eb3 = _fro_sample_err(Ah, Bh, Cb, rows, n)
+            ep = _fro_sample_err(Ah, Bh, Cp, rows, n)
+            _emit({"config": f"C3 SGEMM NN {n}^3 {dist}", "bf16x3_tflops": round(2 * n ** 3 / ms3 / 1e9, 2),
+                   "plain_fp32_tflops": round(2 * n ** 3 / msp / 1e9, 2), "ratio": round(msp / ms3, 3),
+                   "error_bf16x3": eb3, "error_plain": ep})
+            del Ah, Bh, Cb, Cp
+        del A, B, C, C2
+
+
+def _log_uniform(rng, count, lo, hi):
+    return np.exp(rng.uniform(np.log(lo), np.log(hi + 1), (count, 3))).astype(np.int64).clip(lo, hi)
+
+
+def suite_c4():
+    import ctypes
+
+    from coral_amd import device
+    from coral_amd._lib import call
+
+    rng = np.random.default_rng(4)
+    for dt, name, single in ((np.float32, "coral_sgemm_batched", "coral_sgemm"),
+                             (np.float64, "coral_dgemm_batched", "coral_dgemm")):
+        P = 10000
+        dims = _log_uniform(rng, P, 8, 512)
+        sa = dims[:, 0] * dims[:, 2]
+        sb = dims[:, 2] * dims[:, 1]
+        sc = dims[:, 0] * dims[:, 1]
+        oa, ob, oc = (np.concatenate([[0], np.cumsum(s)[:-1]]) for s in (sa, sb, sc))
+        A, B, C = device.DeviceArray(int(sa.sum()), dt), device.DeviceArray(int(sb.sum()), dt), \
+            device.DeviceArray(int(sc.sum()), dt)
+        device.fill_random(A, 41, "normal")
+        device.fill_random(B, 42, "normal")
+        es = np.dtype(dt).itemsize
+        PA, PB, PC = (device.DeviceArray.from_numpy((base.ptr + es * o).astype(np.uint64))
+                      for base, o in ((A, oa), (B, ob), (C, oc)))
+        M, N, K = (device.DeviceArray.from_numpy(dims[:, i].astype(np.int32)) for i in range(3))
+        LDA, LDB, LDC = M, K, M
+        stream = device.Stream()
+
+        def batched():
+            call(name, 0, 0, P, M.ptr, N.ptr, K.ptr, 1.0, PA.ptr, LDA.ptr, PB.ptr, LDB.ptr, 0.0, PC.ptr, LDC.ptr,
+                 stream.handle)
+
+        batched()
+        stream.synchronize()
+        g = ctypes.c_void_p()
+        call("coral_graph_begin_capture", stream.handle)
+        batched()
+        call("coral_graph_end_capture", stream.handle, ctypes.byref(g))
+        e0, e1 = device.Event(), device.Event()
+        call("coral_graph_launch", g.value, stream.handle)
+        stream.synchronize()
+        e0.record(stream)
+        for _ in range(10):
+            call("coral_graph_launch", g.value, stream.handle)
+        e1.record(stream)
+        e1.synchronize()
+        ms_graph = e0.elapsed_ms(e1) / 10
+        outs = C.to_numpy()
+        # per-problem launch loop over the same problems
+        lib_fn = single
+        hp = [(int(m), int(n), int(k), A.ptr + es * int(a), B.ptr + es * int(b), C.ptr + es * int(c))
+              for (m, n, k), a, b, c in zip(dims, oa, ob, oc)]
+        e0.record(stream)
+        for (m, n, k, pa, pb, pc) in hp:
+            call(lib_fn, 0, 0, m, n, k, 1.0, pa, m, pb, k, 0.0, pc, m, stream.handle)
+        e1.record(stream)
+        e1.synchronize()
+        ms_loop = e0.elapsed_ms(e1)
+        loop_out = C.to_numpy()
+        diff = float(np.abs(outs - loop_out).max() / max(np.abs(loop_out).max(), 1e-30))
+        flops = float((2 * dims[:, 0] * dims[:, 1] * dims[:, 2]).sum())
+        _emit({"config": f"C4 batched {'S' if dt == np.float32 else 'D'}GEMM 10000 problems log-U[8,512] NN",
+               "graph_ms": round(ms_graph, 3), "loop_ms": round(ms_loop, 3), "speedup": round(ms_loop / ms_graph, 2),
+               "tflops": round(flops / ms_graph / 1e9, 2), "flops": flops, "max_rel_diff_vs_loop": diff})
+        call("coral_graph_destroy", g.value)
+        stream.destroy()
+    # batched DTRSV: 4096 lower non-unit, n log-U[32,256]
+    ns = np.exp(rng.uniform(np.log(32), np.log(257), 4096)).astype(np.int64).clip(32, 256)
+    mats, xs = [], []
+    for n in ns:
+        Ah = np.diag(rng.uniform(1, 2, n)) + np.tril(rng.uniform(-1, 1, (n, n)) / n, -1)
+        mats.append(np.asfortranarray(Ah).reshape(-1, order="F"))
+        xs.append(rng.uniform(-1, 1, n))
+    oa = np.concatenate([[0], np.cumsum([m.size for m in mats])[:-1]])
+    ox = np.concatenate([[0], np.cumsum([x.size for x in xs])[:-1]])
+    DA = device.DeviceArray.from_numpy(np.concatenate(mats))
+    b0 = np.concatenate(xs)
+    DX = device.DeviceArray.from_numpy(b0)
+    PA = device.DeviceArray.from_numpy((DA.ptr + 8 * oa).astype(np.uint64))
+    PX = device.DeviceArray.from_numpy((DX.ptr + 8 * ox).astype(np.uint64))
+    NS = device.DeviceArray.from_numpy(ns.astype(np.int32))
+    e0, e1 = device.Event(), device.Event()
+    times = []
+    for _ in range(5):
+        DX.upload(b0)
+        device.synchronize()
+        e0.record()
+        call("coral_dtrsv_batched", 1, 0, 1, len(ns), NS.ptr, PA.ptr, NS.ptr, PX.ptr, None)
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_ms(e1))
+    ms_b = statistics.median(times)
+    DX.upload(b0)
+    device.synchronize()
+    e0.record()
+    for n, a_off, x_off in zip(ns, oa, ox):
+        call("coral_dtrsv", 1, 0, 1, int(n), DA.ptr + 8 * int(a_off), int(n), DX.ptr + 8 * int(x_off), 1, None)
+    e1.record()
+    e1.synchronize()
+    ms_l = e0.elapsed_ms(e1)
+    _emit({"config": "C4 batched DTRSV 4096 lower non-unit n log-U[32,256]", "batched_ms": round(ms_b, 3),
+           "loop_ms": round(ms_l, 3), "speedup": round(ms_l / ms_b, 2),
+           "gflops": round(float((ns.astype(float) ** 2).sum()) / ms_b / 1e6, 2)})
+
+
+def suite_c5():
+    from coral_amd import device, synthetic
+    from coral_amd._lib import call
+
+    n = 4096
+    A, B, C = (device.DeviceArray(n * n, np.complex64) for _ in range(3))
+    device.fill_random(A, synthetic.seed_for(5, "A"), "uniform")
+    device.fill_random(B, synthetic.seed_for(5, "B"), "uniform")
+    import ctypes
+
+    one = (ctypes.c_float * 2)(1.0, 0.0)
+    zero = (ctypes.c_float * 2)(0.0, 0.0)
+    Ah, Bh = _host_mat(A, n, n, np.complex64), _host_mat(B, n, n, np.complex64)
+    rows = np.random.default_rng(6).choice(n, 64, replace=False)
+    for name in ("coral_cgemm", "coral_cgemm_3m_bf16x3"):
+        ms = timed_steps(lambda: call(name, 2, 2, n, n, n, one, A.ptr, n, B.ptr, n, zero, C.ptr, n, None), 5, 1)
+        err = _fro_sample_err(Ah, Bh, _host_mat(C, n, n, np.complex64), rows, n, cplx=True, ops=(2, 2))
+        _emit({"config": f"C5 {'CGEMM 4M FP32-MFMA' if name == 'coral_cgemm' else 'CGEMM 3M BF16x3'} CC 4096^3",
+               "ms": round(ms, 3), "tflops": round(8 * n ** 3 / ms / 1e9, 2), "error_vs_fp64": err})
+    del A, B, C, Ah, Bh
+    A, B, C = (device.DeviceArray(n * n, np.complex128) for _ in range(3))
+    device.fill_random(A, synthetic.seed_for(5, "A"), "uniform")
+    device.fill_random(B, synthetic.seed_for(5, "B"), "uniform")
+    oned = (ctypes.c_double * 2)(1.0, 0.0)
+    zerod = (ctypes.c_double * 2)(0.0, 0.0)
+    ms = timed_steps(lambda: call("coral_zgemm", 0, 0, n, n, n, oned, A.ptr, n, B.ptr, n, zerod, C.ptr, n, None), 3, 1)
+    tf = 8 * n ** 3 / ms / 1e9
+    _emit({"config": "C5 ZGEMM NN 4096^3 FP64-MFMA", "ms": round(ms, 3), "tflops": round(tf, 2),
+           "frac_fp64_peak": round(tf / 78.6, 4)})
+    del A, B, C
+    m = 16384
+    Mh = device.DeviceArray(m * m, np.complex64)
+    device.fill_random(Mh, synthetic.seed_for(5, "C"), "uniform")
+    x = device.DeviceArray(m, np.complex64)
+    y = device.DeviceArray(m, np.complex64)
+    device.fill_random(x, synthetic.seed_for(5, "x"), "uniform")
+    ms = timed_steps(lambda: call("coral_chemv", 0, m, one, Mh.ptr, m, x.ptr, 1, zero, y.ptr, 1, None), 10, 2)
+    by = 8.0 * m * (m + 1) / 2 + 3 * 8.0 * m
+    _emit({"config": "C5 CHEMV upper 16384 (stored triangle of a random complex64 matrix)", "us": round(ms * 1e3, 1),
+           "gbps": round(by / ms / 1e6, 1), "frac_hbm": round(by / ms / 1e6 / PEAK_HBM, 4)})
+
+
+def run_suite(which):
+    from coral_amd._lib import load
+
+    lib = load()
+    if lib.coral_init() != 0:
+        raise SystemExit(f"coral_init failed: {lib.coral_last_error().decode()}")
+    table = {"c1": suite_c1, "c2": suite_c2, "c3": suite_c3, "c4": suite_c4, "c5": suite_c5}
+    for w in which:
+        log(f"suite {w} ...")
+        table[w]()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,12 +424,16 @@ def main():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--gemv-n", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--suite", default="", help="comma list of c1,c2,c3,c4,c5: one JSON line per configuration")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {max(args.gpus, world)}).",
               file=sys.stderr)
         sys.exit(2)
+    if args.suite:
+        run_suite([s.strip() for s in args.suite.split(",") if s.strip()])
+        return
 
     from coral_amd import device, synthetic
     from coral_amd._lib import call, load

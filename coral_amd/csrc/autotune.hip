@@ -19,6 +19,13 @@ namespace coral {
 static std::mutex g_tune_mu;
 static std::map<std::string, int> g_table;
 static thread_local int g_forced_tile = -1;
+static int g_version = 0;  // bumped on every table change
+
+// Changes whenever a lookup could return something new (table edits or a forced tile).
+int autotune_version() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return g_version * 16 + (g_forced_tile + 1);
+}
 
 static int bucket(int64_t x) {
   int b = 0;
@@ -53,6 +60,80 @@ struct ForcedTile {
   ~ForcedTile() { g_forced_tile = -1; }
 };
 
+// Batched buckets: a homogeneous batch of P problems of shape (m, n, k) (shared A and
+// B, distinct C blocks) is timed with each tile kind forced; the winner is stored for
+// the (m, n, k) bucket.
+static int tune_batched(const char *routine, int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k,
+                        int32_t *chosen) {
+  const bool dbl = routine[0] == 'd';
+  const size_t es = dbl ? 8 : 4;
+  int64_t P = ((int64_t)1 << 24) / (m * n);
+  P = P < 16 ? 16 : (P > 4096 ? 4096 : P);
+  const int64_t lda = op_a == CORAL_NO_TRANSPOSE ? m : k, ldb = op_b == CORAL_NO_TRANSPOSE ? k : n;
+  void *a = nullptr, *b = nullptr, *c = nullptr, *dims = nullptr, *ptrs = nullptr;
+  CORAL_HIP(hipMalloc(&a, es * (size_t)(m * k)));
+  CORAL_HIP(hipMalloc(&b, es * (size_t)(k * n)));
+  CORAL_HIP(hipMalloc(&c, es * (size_t)(m * n * P)));
+  CORAL_HIP(hipMalloc(&dims, sizeof(int32_t) * 6 * (size_t)P));
+  CORAL_HIP(hipMalloc(&ptrs, sizeof(void *) * 3 * (size_t)P));
+  std::vector<int32_t> hd(6 * (size_t)P);
+  std::vector<void *> hp(3 * (size_t)P);
+  for (int64_t p = 0; p < P; ++p) {
+    hd[p] = (int32_t)m; hd[P + p] = (int32_t)n; hd[2 * P + p] = (int32_t)k;
+    hd[3 * P + p] = (int32_t)lda; hd[4 * P + p] = (int32_t)ldb; hd[5 * P + p] = (int32_t)m;
+    hp[p] = a; hp[P + p] = b; hp[2 * P + p] = (char *)c + es * (size_t)(m * n * p);
+  }
+  CORAL_HIP(hipMemcpy(dims, hd.data(), hd.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  CORAL_HIP(hipMemcpy(ptrs, hp.data(), hp.size() * sizeof(void *), hipMemcpyHostToDevice));
+  if (dbl) {
+    coral_fill_random_f64((double *)a, m * k, 11, 0, nullptr);
+    coral_fill_random_f64((double *)b, k * n, 12, 0, nullptr);
+  } else {
+    coral_fill_random_f32((float *)a, m * k, 11, 0, nullptr);
+    coral_fill_random_f32((float *)b, k * n, 12, 0, nullptr);
+  }
+  const int32_t *dm = (const int32_t *)dims;
+  void **pp = (void **)ptrs;
+  auto run_once = [&]() -> int {
+    if (dbl)
+      return coral_dgemm_batched(op_a, op_b, P, dm, dm + P, dm + 2 * P, 1.0, (const double *const *)pp, dm + 3 * P,
+                                 (const double *const *)(pp + P), dm + 4 * P, 0.0, (double *const *)(pp + 2 * P),
+                                 dm + 5 * P, nullptr);
+    return coral_sgemm_batched(op_a, op_b, P, dm, dm + P, dm + 2 * P, 1.0f, (const float *const *)pp, dm + 3 * P,
+                               (const float *const *)(pp + P), dm + 4 * P, 0.0f, (float *const *)(pp + 2 * P),
+                               dm + 5 * P, nullptr);
+  };
+  hipEvent_t e0, e1;
+  CORAL_HIP(hipEventCreate(&e0));
+  CORAL_HIP(hipEventCreate(&e1));
+  int best = -1, status = CORAL_OK;
+  float best_ms = 1e30f;
+  for (int t = 0; t < batched_menu_size(routine) && status == CORAL_OK; ++t) {
+    ForcedTile ft(t);
+    status = run_once();
+    if (status != CORAL_OK) break;
+    CORAL_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < 3 && status == CORAL_OK; ++r) status = run_once();
+    CORAL_HIP(hipEventRecord(e1, nullptr));
+    CORAL_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CORAL_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) { best_ms = ms; best = t; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a); (void)hipFree(b); (void)hipFree(c); (void)hipFree(dims); (void)hipFree(ptrs);
+  if (status != CORAL_OK) return status;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_table[key_of(routine, op_a == CORAL_CONJUGATE_TRANSPOSE ? 1 : op_a, op_b == CORAL_CONJUGATE_TRANSPOSE ? 1 : op_b,
+                   m, n, k)] = best;
+    ++g_version;
+  }
+  if (chosen) *chosen = best;
+  return CORAL_OK;
+}
+
 }  // namespace coral
 
 using namespace coral;
@@ -75,12 +156,14 @@ int coral_autotune_set(const char *routine, int32_t op_a, int32_t op_b, int64_t 
   if (tile < 0 || tile >= tile_count(routine)) return -7;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_table[key_of(routine, op_a, op_b, m, n, k)] = tile;
+  ++g_version;
   return CORAL_OK;
 }
 
 int coral_autotune_clear(void) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_table.clear();
+  ++g_version;
   return CORAL_OK;
 }
 
@@ -118,6 +201,7 @@ int coral_autotune_load(const char *path) {
   fclose(f);
   std::lock_guard<std::mutex> lk(g_tune_mu);
   for (auto &kv : loaded) g_table[kv.first] = kv.second;
+  ++g_version;
   return CORAL_OK;
 }
 
@@ -128,13 +212,14 @@ int coral_autotune_run(const char *routine, int32_t op_a, int32_t op_b, int64_t 
                        int32_t *chosen_tile) {
   if (!routine) return -1;
   const int ntiles = tile_count(routine);
-  if (ntiles == 0 || strstr(routine, "batched")) return -1;
+  if (ntiles == 0) return -1;
   if (!valid_op(op_a)) return -2;
   if (!valid_op(op_b)) return -3;
   if (m <= 0) return -4;
   if (n <= 0) return -5;
   if (k <= 0) return -6;
   CORAL_TRY(ensure_init());
+  if (strstr(routine, "batched")) return tune_batched(routine, op_a, op_b, m, n, k, chosen_tile);
   const bool dbl = routine[0] == 'd' || routine[0] == 'z';
   const bool cpx = routine[0] == 'c' || routine[0] == 'z';
   const size_t es = (dbl ? 8 : 4) * (cpx ? 2 : 1);
@@ -197,6 +282,7 @@ int coral_autotune_run(const char *routine, int32_t op_a, int32_t op_b, int64_t 
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_table[key_of(routine, op_a, op_b, m, n, k)] = best;
+    ++g_version;
   }
   if (chosen_tile) *chosen_tile = best;
   return CORAL_OK;

@@ -24,78 +24,13 @@
 // permutation invariant as long as A and B use the same (lane, t) -> k map, so
 // every 16-byte vector feeds VE 16x16x4 MFMAs (f32/f64) or one 16x16x32 BF16 MFMA.
 #include "coral_common.hpp"
+#include "mfma.hpp"
 
 #include <algorithm>
 #include <string.h>
 
 namespace coral {
 namespace gemm {
-
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-typedef double f64x2 __attribute__((ext_vector_type(2)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef uint32_t v4u __attribute__((ext_vector_type(4)));  // one 16-byte fragment vector
-
-// ---------------------------------------------------------------------------
-// element traits
-// ---------------------------------------------------------------------------
-struct TF64 {  // FP64 MFMA 16x16x4, DGEMM / ZGEMM
-  using T = double;
-  using acc_t = f64x4;
-  static constexpr int VE = 2, PL = 1, KF = 8;
-  static constexpr bool SPLIT = false;
-  static constexpr bool ROWS_INTERLEAVED = true;
-  __device__ static __forceinline__ void mma(acc_t &c, const v4u *a, const v4u *b) {
-    f64x2 av = __builtin_bit_cast(f64x2, a[0]);
-    f64x2 bv = __builtin_bit_cast(f64x2, b[0]);
-    c = __builtin_amdgcn_mfma_f64_16x16x4f64(av[0], bv[0], c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f64_16x16x4f64(av[1], bv[1], c, 0, 0, 0);
-  }
-};
-
-struct TF32 {  // FP32 MFMA 16x16x4, plain SGEMM / CGEMM 4M
-  using T = float;
-  using acc_t = f32x4;
-  static constexpr int VE = 4, PL = 1, KF = 16;
-  static constexpr bool SPLIT = false;
-  static constexpr bool ROWS_INTERLEAVED = false;
-  __device__ static __forceinline__ void mma(acc_t &c, const v4u *a, const v4u *b) {
-    f32x4 av = __builtin_bit_cast(f32x4, a[0]);
-    f32x4 bv = __builtin_bit_cast(f32x4, b[0]);
-    c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[0], bv[0], c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[1], bv[1], c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[2], bv[2], c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[3], bv[3], c, 0, 0, 0);
-  }
-};
-
-struct TBF3 {  // BF16 hi/lo planes, three 16x16x32 BF16 MFMAs, FP32 accumulation
-  using T = float;
-  using acc_t = f32x4;
-  static constexpr int VE = 8, PL = 2, KF = 32;
-  static constexpr bool SPLIT = true;
-  static constexpr bool ROWS_INTERLEAVED = false;
-  __device__ static __forceinline__ void mma(acc_t &c, const v4u *a, const v4u *b) {
-    bf16x8 ah = __builtin_bit_cast(bf16x8, a[0]);
-    bf16x8 al = __builtin_bit_cast(bf16x8, a[1]);
-    bf16x8 bh = __builtin_bit_cast(bf16x8, b[0]);
-    bf16x8 bl = __builtin_bit_cast(bf16x8, b[1]);
-    // small cross terms first, then the dominant hi*hi product
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
-  }
-};
-
-// IEEE binary32 -> bfloat16, round to nearest even (NaN stays a quiet NaN).
-__device__ __forceinline__ uint32_t bf16_rn_bits(float v) {
-  uint32_t u = __float_as_uint(v);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return u >> 16;
-}
-__device__ __forceinline__ float bf16_bits_to_f32(uint32_t h) { return __uint_as_float(h << 16); }
 
 // ---------------------------------------------------------------------------
 // operand fetchers: value of the packed operand at (r, k) for batch slot z.
@@ -308,29 +243,31 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
 
-  v4u ra[LA], rb[LB];
+  // Pipeline: LDS holds tiles kt (read) and kt+1 (written at the end of iteration
+  // kt); registers hold tiles in flight from HBM, two tiles ahead, in two slots:
+  // slot (t & 1) carries tile t.  Iteration kt computes from LDS, stores tile kt+1
+  // (slot (kt+1)&1, loaded two iterations earlier) into the other LDS stage, then
+  // reuses that slot for tile kt+3.  The loop is unrolled by two so slot indices
+  // stay compile-time constants (register arrays, no scratch).
+  const int64_t KT = g.KT;
+  v4u ra0[LA], rb0[LB], ra1[LA], rb1[LB];
+  auto load_tile = [&](int64_t t, v4u(&ra)[LA], v4u(&rb)[LB]) {
+    const v4u *an = Ag + t * ATV;
+    const v4u *bn = Bg + t * BTV;
 #pragma unroll
-  for (int i = 0; i < LA; ++i) ra[i] = Ag[i * 256];
+    for (int i = 0; i < LA; ++i) ra[i] = an[i * 256];
 #pragma unroll
-  for (int i = 0; i < LB; ++i) rb[i] = Bg[i * 256];
+    for (int i = 0; i < LB; ++i) rb[i] = bn[i * 256];
+  };
+  auto store_tile = [&](int stage, const v4u(&ra)[LA], const v4u(&rb)[LB]) {
+    v4u *dst = lds + stage * (ATV + BTV);
 #pragma unroll
-  for (int i = 0; i < LA; ++i) lds[tid + i * 256] = ra[i];
+    for (int i = 0; i < LA; ++i) dst[tid + i * 256] = ra[i];
 #pragma unroll
-  for (int i = 0; i < LB; ++i) lds[ATV + tid + i * 256] = rb[i];
-  __syncthreads();
-
-  for (int64_t kt = 0; kt < g.KT; ++kt) {
-    const int cur = (int)(kt & 1);
-    const bool more = kt + 1 < g.KT;
-    if (more) {
-      const v4u *an = Ag + (kt + 1) * ATV;
-      const v4u *bn = Bg + (kt + 1) * BTV;
-#pragma unroll
-      for (int i = 0; i < LA; ++i) ra[i] = an[i * 256];
-#pragma unroll
-      for (int i = 0; i < LB; ++i) rb[i] = bn[i * 256];
-    }
-    const v4u *la = lds + cur * (ATV + BTV);
+    for (int i = 0; i < LB; ++i) dst[ATV + tid + i * 256] = rb[i];
+  };
+  auto compute = [&](int stage) {
+    const v4u *la = lds + stage * (ATV + BTV);
     const v4u *lb = la + ATV;
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc) {
@@ -348,14 +285,23 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < FN; ++j) TR::mma(acc[i][j], fa[i], fb[j]);
     }
-    if (more) {
-      v4u *dst = lds + (cur ^ 1) * (ATV + BTV);
-#pragma unroll
-      for (int i = 0; i < LA; ++i) dst[tid + i * 256] = ra[i];
-#pragma unroll
-      for (int i = 0; i < LB; ++i) dst[ATV + tid + i * 256] = rb[i];
-    }
+  };
+  // iteration kt: slot s = (kt + 1) & 1 carries tile kt + 1
+  auto step = [&](int64_t kt, v4u(&ra)[LA], v4u(&rb)[LB]) {
+    compute((int)(kt & 1));
+    if (kt + 1 < KT) store_tile((int)((kt + 1) & 1), ra, rb);
+    if (kt + 3 < KT) load_tile(kt + 3, ra, rb);
     __syncthreads();
+  };
+
+  load_tile(0, ra0, rb0);
+  store_tile(0, ra0, rb0);
+  if (KT > 1) load_tile(1, ra1, rb1);
+  if (KT > 2) load_tile(2, ra0, rb0);
+  __syncthreads();
+  for (int64_t kt = 0; kt < KT; kt += 2) {
+    step(kt, ra1, rb1);                    // stores tile kt+1 (slot 1), refills slot 1 with kt+3
+    if (kt + 1 < KT) step(kt + 1, ra0, rb0);  // stores tile kt+2 (slot 0), refills slot 0 with kt+4
   }
 
   // ---- epilogue ----

@@ -1,0 +1,84 @@
+// MFMA element traits shared by the GEMM kernels (gemm.hip, batched.hip).
+//
+// A "fragment vector" is 16 bytes per lane.  Lane l of a 16x16 MFMA operand holds
+// row (l % 16) and k-values (l / 16) * VE + t, t < VE, with VE = 16 / sizeof(elem);
+// A and B use the same (lane, t) -> k map, so every 16-byte vector feeds VE 16x16x4
+// MFMAs (f32/f64) or one 16x16x32 BF16 MFMA (hi and lo planes: PL = 2).
+// The 16x16 accumulator holds column l % 16 in lane l; register r holds row
+// 4*(l/16)+r for f32/bf16 MFMA and row l/16+4*r for f64 MFMA (ROWS_INTERLEAVED).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace coral {
+namespace gemm {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));  // one 16-byte fragment vector
+
+// ---------------------------------------------------------------------------
+// element traits
+// ---------------------------------------------------------------------------
+struct TF64 {  // FP64 MFMA 16x16x4, DGEMM / ZGEMM
+  using T = double;
+  using acc_t = f64x4;
+  static constexpr int VE = 2, PL = 1, KF = 8;
+  static constexpr bool SPLIT = false;
+  static constexpr bool ROWS_INTERLEAVED = true;
+  __device__ static __forceinline__ void mma(acc_t &c, const v4u *a, const v4u *b) {
+    f64x2 av = __builtin_bit_cast(f64x2, a[0]);
+    f64x2 bv = __builtin_bit_cast(f64x2, b[0]);
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(av[0], bv[0], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(av[1], bv[1], c, 0, 0, 0);
+  }
+};
+
+struct TF32 {  // FP32 MFMA 16x16x4, plain SGEMM / CGEMM 4M
+  using T = float;
+  using acc_t = f32x4;
+  static constexpr int VE = 4, PL = 1, KF = 16;
+  static constexpr bool SPLIT = false;
+  static constexpr bool ROWS_INTERLEAVED = false;
+  __device__ static __forceinline__ void mma(acc_t &c, const v4u *a, const v4u *b) {
+    f32x4 av = __builtin_bit_cast(f32x4, a[0]);
+    f32x4 bv = __builtin_bit_cast(f32x4, b[0]);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[0], bv[0], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[1], bv[1], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[2], bv[2], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[3], bv[3], c, 0, 0, 0);
+  }
+};
+
+struct TBF3 {  // BF16 hi/lo planes, three 16x16x32 BF16 MFMAs, FP32 accumulation
+  using T = float;
+  using acc_t = f32x4;
+  static constexpr int VE = 8, PL = 2, KF = 32;
+  static constexpr bool SPLIT = true;
+  static constexpr bool ROWS_INTERLEAVED = false;
+  __device__ static __forceinline__ void mma(acc_t &c, const v4u *a, const v4u *b) {
+    bf16x8 ah = __builtin_bit_cast(bf16x8, a[0]);
+    bf16x8 al = __builtin_bit_cast(bf16x8, a[1]);
+    bf16x8 bh = __builtin_bit_cast(bf16x8, b[0]);
+    bf16x8 bl = __builtin_bit_cast(bf16x8, b[1]);
+    // small cross terms first, then the dominant hi*hi product
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+  }
+};
+
+// IEEE binary32 -> bfloat16, round to nearest even (NaN stays a quiet NaN).
+__device__ __forceinline__ uint32_t bf16_rn_bits(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+__device__ __forceinline__ float bf16_bits_to_f32(uint32_t h) { return __uint_as_float(h << 16); }
+
+}  // namespace gemm
+}  // namespace coral

@@ -286,6 +286,36 @@ int coral_event_synchronize(void *event) {
   return CORAL_OK;
 }
 
+int coral_graph_begin_capture(coral_stream_t stream) {
+  CORAL_TRY(ensure_init());
+  if (!stream) {
+    set_last_error("graph capture needs a created stream (not the null stream)");
+    return -1;
+  }
+  CORAL_HIP(hipStreamBeginCapture(resolve(stream), hipStreamCaptureModeThreadLocal));
+  return CORAL_OK;
+}
+int coral_graph_end_capture(coral_stream_t stream, void **graph_exec) {
+  if (!graph_exec) return -2;
+  hipGraph_t graph = nullptr;
+  CORAL_HIP(hipStreamEndCapture(resolve(stream), &graph));
+  hipGraphExec_t exec = nullptr;
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+  *graph_exec = (void *)exec;
+  return CORAL_OK;
+}
+int coral_graph_launch(void *graph_exec, coral_stream_t stream) {
+  if (!graph_exec) return -1;
+  CORAL_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, resolve(stream)));
+  return CORAL_OK;
+}
+int coral_graph_destroy(void *graph_exec) {
+  if (graph_exec) CORAL_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+  return CORAL_OK;
+}
+
 int coral_timing_enable(int32_t on) {
   CORAL_TRY(ensure_init());
   g_timing_on = on != 0;

@@ -132,8 +132,17 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
   for (int j = 0; j < b; ++j) {
     // wait for x_j
     if (t == 0) {
-      while (__hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0)
+      // Tickets make this wait finite (block j belongs to a workgroup that started
+      // earlier and waits only on smaller tickets); the spin bound is a backstop
+      // that records an error instead of hanging if that invariant were broken.
+      uint32_t spins = 0;
+      while (__hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
         __builtin_amdgcn_s_sleep(2);
+        if (++spins > (1u << 26)) {
+          atomicExch(&ctrl[nb + 1], 1);
+          break;
+        }
+      }
     }
     __syncthreads();
     __threadfence();
@@ -268,9 +277,10 @@ static int trsv_impl(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const
   hipStream_t s = resolve(stream);
   const int nb = (int)cdiv(n, NB);
   void *ctrl = nullptr;
-  CORAL_TRY(workspace(s, WS_SMALL, sizeof(int) * (size_t)(nb + 1), &ctrl));
-  hipLaunchKernelGGL(zero_ints_kernel, dim3((unsigned)std::min<int64_t>(cdiv(nb + 1, 256), 64)), dim3(256), 0, s,
-                     (int *)ctrl, nb + 1);
+  // ctrl: [0] ticket, [1..nb] block flags, [nb+1] wait-bound error flag
+  CORAL_TRY(workspace(s, WS_SMALL, sizeof(int) * (size_t)(nb + 2), &ctrl));
+  hipLaunchKernelGGL(zero_ints_kernel, dim3((unsigned)std::min<int64_t>(cdiv(nb + 2, 256), 64)), dim3(256), 0, s,
+                     (int *)ctrl, nb + 2);
   CORAL_LAUNCH_CHECK();
   TriView<E> v = make_view(uplo, trans, a, lda, n);
   hipLaunchKernelGGL(trsv_kernel<E>, dim3((unsigned)nb), dim3(256), 0, s, v, x, incx,

@@ -79,6 +79,11 @@ int coral_event_destroy(void *event);
 int coral_event_record(void *event, coral_stream_t stream);
 int coral_event_elapsed_ms(float *ms, void *start, void *stop);
 int coral_event_synchronize(void *event);
+/* hipGraph capture of stream work (e.g. a batched call) and replay. */
+int coral_graph_begin_capture(coral_stream_t stream);
+int coral_graph_end_capture(coral_stream_t stream, void **graph_exec);
+int coral_graph_launch(void *graph_exec, coral_stream_t stream);
+int coral_graph_destroy(void *graph_exec);
 /* Seeded synthetic data on the device (counter-based splitmix64; identical
  * bits to the host generator in coral_amd/synthetic.py for the uniform kinds).
  * kind: 0 = uniform[-1,1), 1 = normal(0,1) (Box-Muller in f64). */
@@ -145,9 +150,6 @@ int coral_dgemm_batched(int32_t op_a, int32_t op_b, int64_t batch,
                         const double *const *a, const int32_t *lda,
                         const double *const *b, const int32_t *ldb, double beta,
                         double *const *c, const int32_t *ldc, coral_stream_t stream);
-/* Host-side planning hint for the batched kernels: the largest m, n, k in the
- * batch (device arrays cannot be inspected without a sync).  0 = unknown. */
-int coral_batched_set_hint(int32_t max_m, int32_t max_n, int32_t max_k);
 
 /* Autotuner (no reference counterpart; stands in for CORAL's compile-time
  * MC/NC/KC/MR/NR constants, coral-aarch64/src/level3/sgemm.rs:92-94).

@@ -16,27 +16,37 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
 
+def _abs_sum(x, y):
+    """sum |x_i| |y_i|: the scale of a dot product's rounding error."""
+    return float(np.sum(np.abs(np.asarray(x).astype(np.complex128)) * np.abs(np.asarray(y).astype(np.complex128))))
+
+
 def test_dot(gpu_lib):
+    # |computed - exact| <= c * u * sqrt(n) * sum |x_i y_i| (random-walk accumulation bound)
     rng = np.random.default_rng(30)
     for n, inc in ((0, 1), (1, 1), (31, 1), (1000, 2), (100003, 1), (1 << 20, 1)):
         x = rng.uniform(-1, 1, (n - 1) * inc + 1 if n else 1)
         y = rng.uniform(-1, 1, (n - 1) * inc + 1 if n else 1)
-        e = ref64.dot_exact(ref64.vec(x, n, inc), ref64.vec(y, n, inc))
-        assert abs(blas.ddot(n, x, inc, y, inc) - e) <= 1e-12 * max(1, np.sqrt(n))
+        xv, yv = ref64.vec(x, n, inc), ref64.vec(y, n, inc)
+        s = max(_abs_sum(xv, yv), 1e-300)
+        r = max(1.0, np.sqrt(n))
+        e = ref64.dot_exact(xv, yv)
+        assert abs(blas.ddot(n, x, inc, y, inc) - e) <= 4 * 2.0 ** -53 * r * s + 1e-300
         xf, yf = x.astype(np.float32), y.astype(np.float32)
         ef = ref64.dot_exact(ref64.vec(xf, n, inc), ref64.vec(yf, n, inc))
-        assert abs(blas.sdot(n, xf, inc, yf, inc) - ef) <= 4e-6 * max(1, np.sqrt(n))
+        assert abs(blas.sdot(n, xf, inc, yf, inc) - ef) <= 4 * 2.0 ** -24 * r * s + 1e-30
         xc = (x + 1j * y).astype(np.complex128)
         yc = (y - 0.5j * x).astype(np.complex128)
+        sc = max(_abs_sum(ref64.vec(xc, n, inc), ref64.vec(yc, n, inc)), 1e-300)
         for name, conj in (("zdotu", False), ("zdotc", True)):
-            r = getattr(blas, name)(n, xc, inc, yc, inc)
+            rr = getattr(blas, name)(n, xc, inc, yc, inc)
             e = ref64.dot_exact(ref64.vec(xc, n, inc), ref64.vec(yc, n, inc), conj=conj)
-            assert abs(complex(*r) - e) <= 1e-12 * max(1, np.sqrt(n)), name
+            assert abs(complex(*rr) - e) <= 8 * 2.0 ** -53 * r * sc + 1e-300, name
         for name, conj in (("cdotu", False), ("cdotc", True)):
-            r = getattr(blas, name)(n, xc.astype(np.complex64), inc, yc.astype(np.complex64), inc)
+            rr = getattr(blas, name)(n, xc.astype(np.complex64), inc, yc.astype(np.complex64), inc)
             e = ref64.dot_exact(ref64.vec(xc.astype(np.complex64), n, inc), ref64.vec(yc.astype(np.complex64), n, inc),
                                 conj=conj)
-            assert abs(complex(*r) - e) <= 4e-6 * max(1, np.sqrt(n)), name
+            assert abs(complex(*rr) - e) <= 8 * 2.0 ** -24 * r * sc + 1e-30, name
 
 
 def test_axpy(gpu_lib):
@@ -75,7 +85,8 @@ def test_nrm2(gpu_lib):
         e = ref64.nrm2(xc)
         assert abs(blas.dznrm2(10001, xc, 1) - e) <= 1e-14 * e
     x = rng.uniform(-1, 1, 3001)
-    assert abs(blas.dnrm2(1000, x, 3) - ref64.nrm2(x[::3])) <= 1e-14 * ref64.nrm2(x[::3])
+    xs = ref64.vec(x, 1000, 3)
+    assert abs(blas.dnrm2(1000, x, 3) - ref64.nrm2(xs)) <= 1e-14 * ref64.nrm2(xs)
     assert blas.dnrm2(0, x, 1) == 0.0
     assert np.isnan(blas.dnrm2(3, np.array([1.0, np.nan, 2.0]), 1))
     assert np.isinf(blas.dnrm2(3, np.array([1.0, np.inf, 2.0]), 1))

@@ -60,8 +60,8 @@ def test_gemv_quick_returns_and_beta_zero(gpu_lib):
     blas.dgemv(0, 3, 2, 1.0, np.ones(6), 3, np.ones(2), 1, 0.0, out, 1)
     assert np.array_equal(out, [2.0, 2.0, 2.0])
     out = y.copy()
-    blas.dgemv(1, 3, 2, 0.0, np.ones(6), 3, np.ones(3), 1, 2.0, out, 1)  # alpha 0: y := beta*y
-    assert out[0] == 2.0 and out[2] == 6.0
+    blas.dgemv(1, 3, 2, 0.0, np.ones(6), 3, np.ones(3), 1, 2.0, out, 1)  # alpha 0: y := beta*y (y has n = 2)
+    assert out[0] == 2.0 and np.isnan(out[1]) and out[2] == 3.0
 
 
 @pytest.mark.parametrize("p", "sdcz")
