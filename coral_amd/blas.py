@@ -90,6 +90,8 @@ cgemm = _gemm("c", "coralh_cgemm")
 zgemm = _gemm("z", "coralh_zgemm")
 sgemm_bf16x3 = _gemm("s", "coralh_sgemm_bf16x3")
 cgemm_3m_bf16x3 = _gemm("c", "coralh_cgemm_3m_bf16x3")
+sgemm_fp16x3 = _gemm("s", "coralh_sgemm_fp16x3")
+cgemm_3m_fp16x3 = _gemm("c", "coralh_cgemm_3m_fp16x3")
 
 
 # ---------------------------------------------------------------- level 2

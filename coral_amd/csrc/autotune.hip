@@ -250,6 +250,12 @@ int coral_autotune_run(const char *routine, int32_t op_a, int32_t op_b, int64_t 
     if (!strcmp(routine, "sgemm_bf16x3"))
       return coral_sgemm_bf16x3(op_a, op_b, m, n, k, 1.f, (float *)a, lda, (float *)b, ldb, 0.f, (float *)c, m,
                                 nullptr);
+    if (!strcmp(routine, "sgemm_fp16x3"))
+      return coral_sgemm_fp16x3(op_a, op_b, m, n, k, 1.f, (float *)a, lda, (float *)b, ldb, 0.f, (float *)c, m,
+                                nullptr);
+    if (!strcmp(routine, "cgemm_3m_fp16x3"))
+      return coral_cgemm_3m_fp16x3(op_a, op_b, m, n, k, one_f, (float *)a, lda, (float *)b, ldb, zero_f, (float *)c,
+                                   m, nullptr);
     if (!strcmp(routine, "cgemm"))
       return coral_cgemm(op_a, op_b, m, n, k, one_f, (float *)a, lda, (float *)b, ldb, zero_f, (float *)c, m, nullptr);
     if (!strcmp(routine, "zgemm"))

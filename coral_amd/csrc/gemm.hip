@@ -95,8 +95,8 @@ struct FetchCplx3M {
 // Row scaling for the split form.  mx[r] holds the IEEE bits of max_k |f(r,k)|
 // (non-negative floats order like their bit patterns, so atomicMax on the bits is
 // an exact, order-independent max; NaN is skipped by fmaxf).  The row exponent is
-// e[r] = floor(log2(mx[r])), so row r scaled by 2^-e[r] has max magnitude in
-// [1, 2); zero or non-finite rows keep e = 0.
+// e[r] = floor(log2(mx[r])); row r is scaled by 2^(SCALE_EXP - e[r]), so its max
+// magnitude lies in [2^SCALE_EXP, 2^(SCALE_EXP+1)); zero or non-finite rows keep e = 0.
 __device__ __forceinline__ int exp_of_max(uint32_t bits) {
   const float mx = __uint_as_float(bits);
   return (mx > 0.f && mx <= 3.402823466e38f) ? ilogbf(mx) : 0;
@@ -132,6 +132,189 @@ __global__ __launch_bounds__(256) void row_absmax_kernel(int64_t R, int64_t K, F
   atomicMax(&mx[z * zstride + r], __float_as_uint(m));
 }
 
+// ---------------------------------------------------------------------------
+// Real operands (the hot path).  One "job" per operand: element (r, k) at
+// p[r*sr + k*sk] with sr == 1 (rows contiguous: op(A) = A, op(B) = B^T) or
+// sk == 1 (k contiguous: op(A) = A^T, op(B) = B).  A and B run in one launch:
+// workgroups [0, a.blocks) take job a, the rest job b.
+//
+// Workgroup blocks are shaped so every global read is a 16-byte load and each
+// wave reads whole 1 KiB (rows contiguous) or 512-byte (k contiguous) runs:
+//   rows contiguous: RW = 64*LV rows x BK k (one K tile), LV = 16 / sizeof(T)
+//   k contiguous:    64 rows x KW = 32*LV k (several K tiles)
+// and consecutive workgroups walk the contiguous direction.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RealJob {
+  v4u *dst;                // packed tiles (pack) / unused (absmax)
+  const T *p;
+  int64_t sr, sk, R, K;
+  int64_t KT;              // K tiles of the packed operand
+  int64_t Rp, Kp;          // padded extents: cdiv(R, br) * br, KT * bk
+  uint32_t *mx;            // split form: max|.| bits per row (absmax writes, pack reads); may be null
+  int br;                  // tile rows
+  int rows_contig;         // sr == 1
+  int vec;                 // base and stride allow 16-byte loads
+  int64_t rblocks, kblocks;
+  int64_t blocks;          // rblocks * kblocks
+};
+
+// 16-byte (or guarded scalar) read of LV consecutive elements along the contiguous index.
+template <typename T, int LV>
+__device__ __forceinline__ void load_run(const RealJob<T> &j, int64_t r, int64_t k, T (&v)[LV]) {
+  if (j.rows_contig) {
+    if (j.vec && r + LV <= j.R && k < j.K) {
+      const v4u x = *reinterpret_cast<const v4u *>(j.p + r + k * j.sk);
+      __builtin_memcpy(v, &x, 16);
+      return;
+    }
+#pragma unroll
+    for (int e = 0; e < LV; ++e) v[e] = (r + e < j.R && k < j.K) ? j.p[r + e + k * j.sk] : T(0);
+  } else {
+    if (j.vec && r < j.R && k + LV <= j.K) {
+      const v4u x = *reinterpret_cast<const v4u *>(j.p + r * j.sr + k);
+      __builtin_memcpy(v, &x, 16);
+      return;
+    }
+#pragma unroll
+    for (int e = 0; e < LV; ++e) v[e] = (r < j.R && k + e < j.K) ? j.p[r * j.sr + k + e] : T(0);
+  }
+}
+
+template <class TR, int BK>
+__global__ __launch_bounds__(256) void pack_real_kernel(const RealJob<typename TR::T> ja,
+                                                        const RealJob<typename TR::T> jb) {
+  using T = typename TR::T;
+  constexpr int LV = 16 / sizeof(T);            // elements per 16-byte load
+  constexpr int VE = TR::VE, PL = TR::PL, KF = TR::KF;
+  constexpr int SV = TR::SPLIT ? VE : LV;       // source elements per fragment vector
+  constexpr int RW = 64 * LV, KW = 32 * LV;
+  static_assert(KW % BK == 0, "k-contiguous block must hold whole K tiles");
+  // LDS rows padded so the fragment gather (lanes 16 apart read k values SV apart)
+  // hits different banks
+  constexpr int PADR = SV == 8 ? 2 : (SV == 4 ? 4 : 8);
+  constexpr int ROWS_LDS = BK * (RW + PADR), KC_LDS = 64 * (KW + LV);
+  __shared__ __attribute__((aligned(16))) T s[ROWS_LDS > KC_LDS ? ROWS_LDS : KC_LDS];
+
+  const bool first = blockIdx.x < ja.blocks;
+  const RealJob<T> &j = first ? ja : jb;
+  const int64_t blk = first ? (int64_t)blockIdx.x : (int64_t)blockIdx.x - ja.blocks;
+  const int t = threadIdx.x;
+  int64_t r0, k0;
+  int rspan, kspan, ld;
+  if (j.rows_contig) {
+    r0 = (blk % j.rblocks) * RW;
+    k0 = (blk / j.rblocks) * BK;
+    rspan = RW; kspan = BK; ld = RW + PADR;
+#pragma unroll
+    for (int c = 0; c < BK / 4; ++c) {
+      const int kl = (t >> 6) + 4 * c, rl = LV * (t & 63);
+      T v[LV];
+      load_run<T, LV>(j, r0 + rl, k0 + kl, v);
+#pragma unroll
+      for (int e = 0; e < LV; ++e) s[kl * ld + rl + e] = v[e];  // s[k][r]
+    }
+  } else {
+    k0 = (blk % j.kblocks) * KW;
+    r0 = (blk / j.kblocks) * 64;
+    rspan = 64; kspan = KW; ld = KW + LV;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int rl = (t >> 5) + 8 * c, kl = LV * (t & 31);
+      T v[LV];
+      load_run<T, LV>(j, r0 + rl, k0 + kl, v);
+      v4u x;
+      __builtin_memcpy(&x, v, 16);
+      *reinterpret_cast<v4u *>(&s[rl * ld + kl]) = x;  // s[r][k], 16-byte aligned rows
+    }
+  }
+  __syncthreads();
+
+  const int RG = j.br / 16;
+  const int units = (rspan / 16) * (kspan / KF) * 64;
+  for (int u = t; u < units; u += 256) {
+    const int lane = u & 63, rest = u >> 6;
+    const int rgl = rest % (rspan / 16), kcl = rest / (rspan / 16);
+    const int rl = rgl * 16 + (lane & 15), kl = kcl * KF + (lane >> 4) * SV;
+    const int64_t gr = r0 + rl, gk = k0 + kl;
+    if (gr >= j.Rp || gk >= j.Kp) continue;
+    T v[SV];
+    if (j.rows_contig) {
+#pragma unroll
+      for (int e = 0; e < SV; ++e) v[e] = s[(kl + e) * ld + rl];
+    } else {
+#pragma unroll
+      for (int e = 0; e < SV; e += LV) {
+        const v4u x = *reinterpret_cast<const v4u *>(&s[rl * ld + kl + e]);
+        __builtin_memcpy(&v[e], &x, 16);
+      }
+    }
+    const int64_t tile = (gr / j.br) * j.KT + gk / BK;
+    const int rg = (int)(gr % j.br) / 16, kc = (int)(gk % BK) / KF;
+    v4u *out = j.dst + tile * ((int64_t)(BK / KF) * RG * 64 * PL) + ((int64_t)(kc * RG + rg) * PL) * 64 +
+               ((gr & 15) + 16 * ((gk % KF) / SV));
+    if constexpr (!TR::SPLIT) {
+      out[0] = __builtin_bit_cast(v4u, v);
+    } else {
+      const int sc = (gr < j.R && j.mx) ? TR::SCALE_EXP - exp_of_max(j.mx[gr]) : 0;
+      uint32_t hi[VE], lo[VE];
+#pragma unroll
+      for (int e = 0; e < VE; ++e) TR::split(ldexpf((float)v[e], sc), hi[e], lo[e]);
+      v4u vh, vl;
+      vh.x = hi[0] | (hi[1] << 16); vh.y = hi[2] | (hi[3] << 16);
+      vh.z = hi[4] | (hi[5] << 16); vh.w = hi[6] | (hi[7] << 16);
+      vl.x = lo[0] | (lo[1] << 16); vl.y = lo[2] | (lo[3] << 16);
+      vl.z = lo[4] | (lo[5] << 16); vl.w = lo[6] | (lo[7] << 16);
+      out[0] = vh;
+      out[64] = vl;
+    }
+  }
+}
+
+// max_k |x(r, k)| per row for the split form (FP32 operands), both jobs in one
+// launch; mx must hold zeros.  Rows contiguous: a workgroup covers 256 rows x 128 k
+// (lanes take 4 rows each with 16-byte loads, the 4 waves split k), then one
+// atomicMax per row.  k contiguous: one wave per row and 4096-k chunk.
+__global__ __launch_bounds__(256) void absmax_real_kernel(const RealJob<float> ja, const RealJob<float> jb) {
+  constexpr int KCH_ROWS = 128, KCH_K = 4096;
+  __shared__ float red[4][256];
+  const bool first = blockIdx.x < ja.blocks;
+  const RealJob<float> &j = first ? ja : jb;
+  const int64_t blk = first ? (int64_t)blockIdx.x : (int64_t)blockIdx.x - ja.blocks;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (j.rows_contig) {
+    const int64_t r0 = (blk % j.rblocks) * 256, k0 = (blk / j.rblocks) * KCH_ROWS;
+    const int64_t r = r0 + 4 * lane;
+    float m[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 8
+    for (int c = 0; c < KCH_ROWS / 4; ++c) {
+      float v[4];
+      load_run<float, 4>(j, r, k0 + w + 4 * c, v);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) m[e] = fmaxf(m[e], fabsf(v[e]));
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) red[w][4 * lane + e] = m[e];
+    __syncthreads();
+    const float mm = fmaxf(fmaxf(red[0][t], red[1][t]), fmaxf(red[2][t], red[3][t]));
+    if (r0 + t < j.R) atomicMax(&j.mx[r0 + t], __float_as_uint(mm));
+  } else {
+    const int64_t r = (blk % j.rblocks) * 4 + w, k0 = (blk / j.rblocks) * KCH_K;
+    float m = 0.f;
+    if (r < j.R) {
+#pragma unroll 4
+      for (int c = 0; c < KCH_K / 256; ++c) {
+        float v[4];
+        load_run<float, 4>(j, r, k0 + 256 * c + 4 * lane, v);
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))));
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if (lane == 0 && r < j.R) atomicMax(&j.mx[r], __float_as_uint(m));
+  }
+}
+
 // One thread writes the PL vectors (16 bytes each) of one (tile, kc, rg, lane) unit.
 template <class TR, int BR, int BK, class F>
 __global__ void pack_kernel(v4u *__restrict__ dst, int64_t R, int64_t K, int64_t KT, int64_t units,
@@ -160,15 +343,12 @@ __global__ void pack_kernel(v4u *__restrict__ dst, int64_t R, int64_t K, int64_t
     out[0] = __builtin_bit_cast(v4u, v);
   } else {
     int sc = 0;
-    if (r < R && exps) sc = -exp_of_max(exps[z * exp_zstride + r]);
+    if (r < R && exps) sc = TR::SCALE_EXP - exp_of_max(exps[z * exp_zstride + r]);
     uint32_t hi[VE], lo[VE];
 #pragma unroll
     for (int t = 0; t < VE; ++t) {
-      float x = (r < R && k0 + t < K) ? (float)f(r, k0 + t, z) : 0.f;
-      x = ldexpf(x, sc);  // exact power-of-two scaling
-      uint32_t h = bf16_rn_bits(x);
-      hi[t] = h;
-      lo[t] = bf16_rn_bits(x - bf16_bits_to_f32(h));  // x - hi is exact in fp32
+      const float x = (r < R && k0 + t < K) ? (float)f(r, k0 + t, z) : 0.f;
+      TR::split(ldexpf(x, sc), hi[t], lo[t]);  // power-of-two scaling is exact
     }
     v4u vh, vl;
     vh.x = hi[0] | (hi[1] << 16); vh.y = hi[2] | (hi[3] << 16);
@@ -199,9 +379,84 @@ struct GemmArgs {
   int64_t re_zstride, ce_zstride;
 };
 
+// tile id -> (tm, tn): spread consecutive ids over the 8 XCDs so neighbours land
+// on one XCD, then walk column groups of 8 tile rows (L2 reuse of B panels).
+__device__ __forceinline__ void map_tile(int bid, int tiles_m, int tiles_n, int &tm, int &tn) {
+  const int total = tiles_m * tiles_n;
+  int w = bid;
+  if ((total & 7) == 0) w = (w & 7) * (total >> 3) + (w >> 3);
+  constexpr int GROUP = 8;
+  const int per_group = GROUP * tiles_n;
+  const int grp = w / per_group;
+  const int first_m = grp * GROUP;
+  const int gsz = min(tiles_m - first_m, GROUP);
+  const int win = w - grp * per_group;
+  tm = first_m + win % gsz;
+  tn = win / gsz;
+}
+
+// Writes a wave's FM x FN accumulator blocks; row_base/col_base is the wave's
+// first output row/column (in the real product's index space).
+template <class TR, int EPI, int FM, int FN>
+__device__ __forceinline__ void epilogue(const typename TR::acc_t (&acc)[FM][FN], const GemmArgs &g, int64_t row_base,
+                                         int64_t col_base, int lane, int z) {
+  using T = typename TR::T;
+  const T ar = (T)g.ar, ai = (T)g.ai, br = (T)g.br, bi = (T)g.bi;
+  T *C = reinterpret_cast<T *>(g.C);
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int64_t col = col_base + j * 16 + (lane & 15);
+      // 16x16 accumulator: lane l holds column l%16; register r holds row 4*(l/16)+r
+      // (f32/bf16/f16 MFMA) or row l/16+4*r (f64 MFMA).
+      const int64_t row0 = row_base + i * 16 + (TR::ROWS_INTERLEAVED ? (lane >> 4) : 4 * (lane >> 4));
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = row0 + (TR::ROWS_INTERLEAVED ? 4 * r : r);
+        T v = acc[i][j][r];
+        if constexpr (EPI == EPI_CPLX) {
+          T pv = __shfl_xor(v, 1, 64);
+          const bool odd = lane & 1;
+          const T re = odd ? pv : v, im = odd ? v : pv;
+          const int64_t cj = col >> 1;
+          if (row < g.M && col < g.N) {
+            T *cp = C + 2 * (row + cj * g.ldc);
+            T ore = ar * re - ai * im, oim = ar * im + ai * re;
+            if (!g.beta_zero) {
+              T cr = cp[0], ci = cp[1];
+              ore += br * cr - bi * ci;
+              oim += br * ci + bi * cr;
+            }
+            cp[odd ? 1 : 0] = odd ? oim : ore;
+          }
+        } else {
+          if (row < g.M && col < g.N) {
+            if constexpr (TR::SPLIT) {
+              int sh = 0;
+              if (g.row_exp) sh += exp_of_max(g.row_exp[z * g.re_zstride + row]) - TR::SCALE_EXP;
+              if (g.col_exp) sh += exp_of_max(g.col_exp[z * g.ce_zstride + col]) - TR::SCALE_EXP;
+              v = ldexpf(v, sh);
+            }
+            if constexpr (EPI == EPI_STORE_Z) {
+              C[z * g.c_zstride + row + col * g.ldc] = v;
+            } else {
+              T *cp = C + row + col * g.ldc;
+              T o = ar * v;
+              if (!g.beta_zero) o += br * (*cp);
+              *cp = o;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// Register-staged kernel: 4 waves in a 2 x 2 arrangement, packed tiles travel
+// HBM -> registers -> LDS.
 template <class TR, int BM, int BN, int BK, int EPI>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
-  using T = typename TR::T;
   using acc_t = typename TR::acc_t;
   constexpr int PL = TR::PL, KF = TR::KF;
   constexpr int KC = BK / KF;
@@ -219,20 +474,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   const int wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int z = blockIdx.z;
-
-  // tile id -> (tm, tn): spread consecutive ids over the 8 XCDs so neighbours
-  // land on one XCD, then walk column groups of 8 tile rows (L2 reuse of B panels).
-  int total = g.tiles_m * g.tiles_n;
-  int w = blockIdx.x;
-  if ((total & 7) == 0) w = (w & 7) * (total >> 3) + (w >> 3);
-  const int GROUP = 8;
-  int per_group = GROUP * g.tiles_n;
-  int grp = w / per_group;
-  int first_m = grp * GROUP;
-  int gsz = min(g.tiles_m - first_m, GROUP);
-  int win = w - grp * per_group;
-  int tm = first_m + win % gsz;
-  int tn = win / gsz;
+  int tm, tn;
+  map_tile(blockIdx.x, g.tiles_m, g.tiles_n, tm, tn);
 
   const v4u *Ag = g.Ap + z * g.a_zstride + (int64_t)tm * g.KT * ATV + tid;
   const v4u *Bg = g.Bp + z * g.b_zstride + (int64_t)tn * g.KT * BTV + tid;
@@ -303,58 +546,142 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     step(kt, ra1, rb1);                    // stores tile kt+1 (slot 1), refills slot 1 with kt+3
     if (kt + 1 < KT) step(kt + 1, ra0, rb0);  // stores tile kt+2 (slot 0), refills slot 0 with kt+4
   }
+  epilogue<TR, EPI, FM, FN>(acc, g, (int64_t)tm * BM + wm * WM, (int64_t)tn * BN + wn * WN, lane, z);
+}
 
-  // ---- epilogue ----
-  const T ar = (T)g.ar, ai = (T)g.ai, br = (T)g.br, bi = (T)g.bi;
-  T *C = reinterpret_cast<T *>(g.C);
+#define CORAL_LDS_PTR(p) ((__attribute__((address_space(3))) void *)(p))
+
+// Direct-to-LDS kernel: WVM x WVN waves; packed tiles go HBM -> LDS with
+// global_load_lds_dwordx4 (lane l of a wave writes 16 bytes at the wave's base +
+// 16 l, which is exactly the fragment order of the packed tile), so no registers
+// carry tiles in flight.  Two LDS stages in two separate arrays (the compiler can
+// then tell a ds_read of one stage from the DMA into the other and does not wait
+// on the DMA); one barrier per K tile: iteration kt issues the DMA of tile kt+1,
+// computes tile kt, then waits for its own DMA and meets the other waves.
+//
+// SCH selects the instruction schedule of one K tile: 0 leaves it to the compiler;
+// 1 issues every fragment read of the tile before the first MFMA (a scheduling
+// fence keeps them there), so MFMAs start as soon as their own fragments land
+// while later reads are still in flight; 2 pipelines the reads row by row.
+template <class TR, int BM, int BN, int BK, int WVM, int WVN, int SCH, int EPI>
+__global__ __launch_bounds__(64 * WVM * WVN) void gemm_dma_kernel(GemmArgs g) {
+  using acc_t = typename TR::acc_t;
+  constexpr int NT = 64 * WVM * WVN;
+  constexpr int PL = TR::PL, KF = TR::KF;
+  constexpr int KC = BK / KF;
+  constexpr int RGA = BM / 16, RGB = BN / 16;
+  constexpr int ATV = KC * RGA * PL * 64;
+  constexpr int BTV = KC * RGB * PL * 64;
+  constexpr int LA = ATV / NT, LB = BTV / NT;
+  static_assert(ATV % NT == 0 && BTV % NT == 0, "tile must split evenly over the workgroup");
+  constexpr int WM = BM / WVM, WN = BN / WVN, FM = WM / 16, FN = WN / 16;
+  static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile must be a multiple of 16");
+
+  __shared__ v4u st0[ATV + BTV];
+  __shared__ v4u st1[ATV + BTV];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave % WVM, wn = wave / WVM;
+  const int z = blockIdx.z;
+  int tm, tn;
+  map_tile(blockIdx.x, g.tiles_m, g.tiles_n, tm, tn);
+
+  // wave-uniform bases (scalar registers); each lane adds its own 16-byte slot, so
+  // the DMA uses the saddr + voffset form and costs no 64-bit VGPR address pairs
+  const v4u *Ag = g.Ap + z * g.a_zstride + (int64_t)tm * g.KT * ATV;
+  const v4u *Bg = g.Bp + z * g.b_zstride + (int64_t)tn * g.KT * BTV;
+  const int wbase = wave * 64;  // this wave's first vector within each NT-vector row
+
+  acc_t acc[FM][FN];
 #pragma unroll
-  for (int i = 0; i < FM; ++i) {
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int64_t col = (int64_t)tn * BN + wn * WN + j * 16 + (lane & 15);
-      // 16x16 accumulator: lane l holds column l%16; register r holds row 4*(l/16)+r
-      // (f32/bf16 MFMA) or row l/16+4*r (f64 MFMA).
-      const int64_t row0 = (int64_t)tm * BM + wm * WM + i * 16 + (TR::ROWS_INTERLEAVED ? (lane >> 4) : 4 * (lane >> 4));
+    for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
+
+  auto dma = [&](v4u *st, int64_t t) {
+    const v4u *an = Ag + t * ATV;
+    const v4u *bn = Bg + t * BTV;
+    const uint32_t lane_off = (uint32_t)tid * 16u;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t row = row0 + (TR::ROWS_INTERLEAVED ? 4 * r : r);
-        T v = acc[i][j][r];
-        if constexpr (EPI == EPI_CPLX) {
-          T pv = __shfl_xor(v, 1, 64);
-          const bool odd = lane & 1;
-          const T re = odd ? pv : v, im = odd ? v : pv;
-          const int64_t cj = col >> 1;
-          if (row < g.M && col < g.N) {
-            T *cp = C + 2 * (row + cj * g.ldc);
-            T ore = ar * re - ai * im, oim = ar * im + ai * re;
-            if (!g.beta_zero) {
-              T cr = cp[0], ci = cp[1];
-              ore += br * cr - bi * ci;
-              oim += br * ci + bi * cr;
-            }
-            cp[odd ? 1 : 0] = odd ? oim : ore;
+    for (int i = 0; i < LA; ++i)
+      __builtin_amdgcn_global_load_lds((const void *)((const char *)(an + i * NT) + lane_off),
+                                       CORAL_LDS_PTR(st + i * NT + wbase), 16, 0, 0);
+#pragma unroll
+    for (int i = 0; i < LB; ++i)
+      __builtin_amdgcn_global_load_lds((const void *)((const char *)(bn + i * NT) + lane_off),
+                                       CORAL_LDS_PTR(st + ATV + i * NT + wbase), 16, 0, 0);
+  };
+  auto compute = [&](const v4u *st) {
+    const v4u *la = st;
+    const v4u *lb = st + ATV;
+    if constexpr (SCH == 2) {
+      // software pipeline over the rows of the wave tile: the fragment of row i+1 is
+      // read while the MFMAs of row i run; the group barriers pin that order
+      // (0x100 = LDS reads, 0x008 = MFMA).
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc) {
+        v4u fb[FN][PL], fa[2][PL];
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int p = 0; p < PL; ++p) fb[j][p] = lb[((kc * RGB + wn * FN + j) * PL + p) * 64 + lane];
+#pragma unroll
+        for (int p = 0; p < PL; ++p) fa[0][p] = la[((kc * RGA + wm * FM) * PL + p) * 64 + lane];
+        __builtin_amdgcn_sched_group_barrier(0x100, FN * PL + PL, 0);
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+          if (i + 1 < FM) {
+#pragma unroll
+            for (int p = 0; p < PL; ++p)
+              fa[(i + 1) & 1][p] = la[((kc * RGA + wm * FM + i + 1) * PL + p) * 64 + lane];
+            __builtin_amdgcn_sched_group_barrier(0x100, PL, 0);
           }
-        } else {
-          if (row < g.M && col < g.N) {
-            if constexpr (TR::SPLIT) {
-              int sh = 0;
-              if (g.row_exp) sh += exp_of_max(g.row_exp[z * g.re_zstride + row]);
-              if (g.col_exp) sh += exp_of_max(g.col_exp[z * g.ce_zstride + col]);
-              v = ldexpf(v, sh);
-            }
-            if constexpr (EPI == EPI_STORE_Z) {
-              C[z * g.c_zstride + row + col * g.ldc] = v;
-            } else {
-              T *cp = C + row + col * g.ldc;
-              T o = ar * v;
-              if (!g.beta_zero) o += br * (*cp);
-              *cp = o;
-            }
-          }
+#pragma unroll
+          for (int j = 0; j < FN; ++j) TR::mma(acc[i][j], fa[i & 1], fb[j]);
+          __builtin_amdgcn_sched_group_barrier(0x008, FN * TR::NMMA, 0);
         }
       }
+      return;
+    }
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) {
+      v4u fa[FM][PL], fb[FN][PL];
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int p = 0; p < PL; ++p) fb[j][p] = lb[((kc * RGB + wn * FN + j) * PL + p) * 64 + lane];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int p = 0; p < PL; ++p) fa[i][p] = la[((kc * RGA + wm * FM + i) * PL + p) * 64 + lane];
+      if constexpr (SCH == 1) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) TR::mma(acc[i][j], fa[i], fb[j]);
+    }
+  };
+  auto wait_and_meet = [&]() {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+  };
+
+  const int64_t KT = g.KT;
+  dma(st0, 0);
+  wait_and_meet();
+  for (int64_t kt = 0; kt < KT; kt += 2) {
+    if (kt + 1 < KT) dma(st1, kt + 1);
+    compute(st0);
+    wait_and_meet();
+    if (kt + 1 < KT) {
+      if (kt + 2 < KT) dma(st0, kt + 2);
+      compute(st1);
+      wait_and_meet();
     }
   }
+  epilogue<TR, EPI, FM, FN>(acc, g, (int64_t)tm * BM + wm * WM, (int64_t)tn * BN + wn * WN, lane, z);
 }
 
 // C := beta*C for the quick return (alpha == 0 or k == 0), never reading C when beta == 0.
@@ -403,15 +730,23 @@ __global__ void combine_3m_kernel(const float *Tz, int64_t m, int64_t n, float *
 struct TileShape { int bm, bn, bk; };
 
 // Tile menus per element kind.  Index = tile id stored by the autotuner.
-template <int BM_, int BN_, int BK_> struct Shape { static constexpr int bm = BM_, bn = BN_, bk = BK_; };
+// WVM x WVN > 0 selects the direct-to-LDS kernel with that wave arrangement; 0 x 0
+// the register-staged kernel (2 x 2 waves).
+template <int BM_, int BN_, int BK_, int WVM_ = 0, int WVN_ = 0, int SCH_ = 0> struct Shape {
+  static constexpr int bm = BM_, bn = BN_, bk = BK_, wvm = WVM_, wvn = WVN_, sch = SCH_;
+};
 template <class... S> struct ShapeList {};
 
 // Tile menus per element kind; the index is the tile id the autotuner stores.
 using MenuF64 = ShapeList<Shape<128, 128, 16>, Shape<64, 64, 16>, Shape<128, 128, 32>>;
 using MenuF32 = ShapeList<Shape<128, 128, 32>, Shape<64, 64, 32>, Shape<256, 128, 32>, Shape<256, 256, 32>,
-                          Shape<128, 128, 64>>;
+                          Shape<128, 128, 64>, Shape<256, 256, 32, 2, 2>, Shape<256, 256, 32, 2, 4>,
+                          Shape<128, 128, 64, 2, 2>, Shape<256, 128, 32, 2, 2>, Shape<256, 256, 32, 2, 4, 1>,
+                          Shape<256, 256, 32, 4, 2, 1>, Shape<256, 256, 32, 2, 4, 2>, Shape<256, 256, 32, 4, 2, 2>>;
 using MenuBF3 = ShapeList<Shape<128, 128, 32>, Shape<64, 64, 32>, Shape<256, 128, 32>, Shape<256, 256, 32>,
-                          Shape<128, 128, 64>>;
+                          Shape<128, 128, 64>, Shape<256, 256, 32, 2, 2>, Shape<256, 256, 32, 2, 4>,
+                          Shape<128, 128, 64, 2, 2>, Shape<256, 128, 32, 2, 2>, Shape<256, 256, 32, 2, 4, 1>,
+                          Shape<256, 256, 32, 4, 2, 1>, Shape<256, 256, 32, 2, 4, 2>, Shape<256, 256, 32, 4, 2, 2>>;
 
 template <class L> struct MenuTable;
 template <class... S> struct MenuTable<ShapeList<S...>> {
@@ -426,22 +761,30 @@ template <class TR> struct Menu;
 template <> struct Menu<TF64> : MenuTable<MenuF64> { using list = MenuF64; };
 template <> struct Menu<TF32> : MenuTable<MenuF32> { using list = MenuF32; };
 template <> struct Menu<TBF3> : MenuTable<MenuBF3> { using list = MenuBF3; };
+template <> struct Menu<TF16X3> : MenuTable<MenuBF3> { using list = MenuBF3; };
 
-template <class TR, int BM, int BN, int BK, int EPI>
+template <class TR, int BM, int BN, int BK, int WVM, int WVN, int SCH, int EPI>
 static int launch_gemm(const GemmArgs &g, int zdim, hipStream_t s) {
-  constexpr int KC = BK / TR::KF;
-  constexpr size_t lds_bytes = 2 * (size_t)KC * ((BM + BN) / 16) * TR::PL * 64 * 16;
-  auto kfn = gemm_kernel<TR, BM, BN, BK, EPI>;
-  static bool attr_set = false;
-  if (!attr_set) {
-    CORAL_HIP(hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
-    attr_set = true;
-  }
   dim3 grid((unsigned)(g.tiles_m * g.tiles_n), 1, (unsigned)zdim);
-  const int slot = timing_begin(TK_GEMM, s);
-  hipLaunchKernelGGL(kfn, grid, dim3(256), lds_bytes, s, g);
-  CORAL_LAUNCH_CHECK();
-  timing_end(slot, s);
+  if constexpr (WVM > 0) {
+    const int slot = timing_begin(TK_GEMM, s);
+    hipLaunchKernelGGL((gemm_dma_kernel<TR, BM, BN, BK, WVM, WVN, SCH, EPI>), grid, dim3(64 * WVM * WVN), 0, s, g);
+    CORAL_LAUNCH_CHECK();
+    timing_end(slot, s);
+  } else {
+    constexpr int KC = BK / TR::KF;
+    constexpr size_t lds_bytes = 2 * (size_t)KC * ((BM + BN) / 16) * TR::PL * 64 * 16;
+    auto kfn = gemm_kernel<TR, BM, BN, BK, EPI>;
+    static bool attr_set = false;
+    if (!attr_set) {
+      CORAL_HIP(hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+      attr_set = true;
+    }
+    const int slot = timing_begin(TK_GEMM, s);
+    hipLaunchKernelGGL(kfn, grid, dim3(256), lds_bytes, s, g);
+    CORAL_LAUNCH_CHECK();
+    timing_end(slot, s);
+  }
   return CORAL_OK;
 }
 
@@ -453,7 +796,7 @@ static int dispatch_tile(int, const GemmArgs &, int, hipStream_t, ShapeList<>) {
 }
 template <class TR, int EPI, class S, class... R>
 static int dispatch_tile(int idx, const GemmArgs &g, int zdim, hipStream_t s, ShapeList<S, R...>) {
-  if (idx == 0) return launch_gemm<TR, S::bm, S::bn, S::bk, EPI>(g, zdim, s);
+  if (idx == 0) return launch_gemm<TR, S::bm, S::bn, S::bk, S::wvm, S::wvn, S::sch, EPI>(g, zdim, s);
   return dispatch_tile<TR, EPI>(idx - 1, g, zdim, s, ShapeList<R...>{});
 }
 
@@ -479,6 +822,73 @@ static int launch_pack_t(v4u *dst, int64_t R, int64_t K, F f, const uint32_t *ex
     CORAL_LAUNCH_CHECK();
     return CORAL_OK;
   }
+}
+
+// Job for one real operand: pack (fragment-ordered tiles of br x bk) or absmax.
+template <typename T>
+static RealJob<T> real_job(const FetchReal<T> &f, int64_t R, int64_t K, int br, int bk, v4u *dst, uint32_t *mx,
+                           bool absmax) {
+  constexpr int LV = 16 / sizeof(T);
+  RealJob<T> j{};
+  j.dst = dst;
+  j.p = f.p;
+  j.sr = f.sr;
+  j.sk = f.sk;
+  j.R = R;
+  j.K = K;
+  j.KT = cdiv(K, bk);
+  j.Rp = cdiv(R, br) * br;
+  j.Kp = j.KT * bk;
+  j.mx = mx;
+  j.br = br;
+  j.rows_contig = f.sr == 1;
+  const int64_t stride = j.rows_contig ? f.sk : f.sr;
+  j.vec = ((uintptr_t)f.p % 16 == 0) && (stride % LV == 0);
+  if (absmax) {  // shapes of absmax_real_kernel
+    j.rblocks = j.rows_contig ? cdiv(R, 256) : cdiv(R, 4);
+    j.kblocks = j.rows_contig ? cdiv(K, 128) : cdiv(K, 4096);
+  } else {       // shapes of pack_real_kernel
+    j.rblocks = j.rows_contig ? cdiv(j.Rp, 64 * LV) : cdiv(j.Rp, 64);
+    j.kblocks = j.rows_contig ? j.KT : cdiv(j.Kp, 32 * LV);
+  }
+  j.blocks = j.rblocks * j.kblocks;
+  return j;
+}
+
+template <class TR, int BK>
+static int launch_pack_real_bk(const RealJob<typename TR::T> &ja, const RealJob<typename TR::T> &jb, hipStream_t s) {
+  if constexpr (BK % TR::KF != 0) {
+    set_last_error("pack: k tile %d is not a multiple of %d", BK, TR::KF);
+    return CORAL_ERR_UNSUPPORTED;
+  } else {
+    hipLaunchKernelGGL((pack_real_kernel<TR, BK>), dim3((unsigned)(ja.blocks + jb.blocks)), dim3(256), 0, s, ja, jb);
+    CORAL_LAUNCH_CHECK();
+    return CORAL_OK;
+  }
+}
+
+// Packs op(A) and op(B) in one launch.
+template <class TR>
+static int launch_pack_real(const TileShape &ts, v4u *pa, v4u *pb, int64_t M, int64_t N, int64_t K,
+                            const FetchReal<typename TR::T> &fa, const FetchReal<typename TR::T> &fb,
+                            uint32_t *rexp, uint32_t *cexp, hipStream_t s) {
+  const auto ja = real_job(fa, M, K, ts.bm, ts.bk, pa, rexp, false);
+  const auto jb = real_job(fb, N, K, ts.bn, ts.bk, pb, cexp, false);
+  if (ts.bk == 16) return launch_pack_real_bk<TR, 16>(ja, jb, s);
+  if (ts.bk == 32) return launch_pack_real_bk<TR, 32>(ja, jb, s);
+  if (ts.bk == 64) return launch_pack_real_bk<TR, 64>(ja, jb, s);
+  set_last_error("pack: no kernel for k tile %d", ts.bk);
+  return CORAL_ERR_UNSUPPORTED;
+}
+
+// Row maxima of op(A) (m rows) and op(B) (n rows) into ex[0, m) and ex[m, m+n).
+static int launch_absmax_real(const FetchReal<float> &fa, const FetchReal<float> &fb, int64_t m, int64_t n,
+                              int64_t k, uint32_t *ex, hipStream_t s) {
+  const auto ja = real_job(fa, m, k, 16, 16, nullptr, ex, true);
+  const auto jb = real_job(fb, n, k, 16, 16, nullptr, ex + m, true);
+  hipLaunchKernelGGL(absmax_real_kernel, dim3((unsigned)(ja.blocks + jb.blocks)), dim3(256), 0, s, ja, jb);
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
 }
 
 template <class TR, class F>
@@ -542,8 +952,13 @@ static int default_tile(int64_t M, int64_t N, int64_t K) {
     // FP64 MFMA is slow per instruction; 64x64 tiles keep 4 waves per SIMD resident
     // (measured 58.5 vs 47.9 TFLOP/s for 128x128 at 8192^3)
     return 1;
-  } else if constexpr (std::is_same<TR, TBF3>::value) {
-    if (tiles(t[3]) >= cus) return 3;  // 256x256: fewest LDS bytes per MFMA
+  } else {
+    // 256x256 tiles, 8 waves, direct-to-LDS loads: fewest LDS bytes per MFMA.  At
+    // 8192^3 (warm clocks): BF16x3 492 TFLOP/s with the row-pipelined schedule
+    // (id 11) vs 448 compiler-scheduled (id 6) and 361 register-staged (id 3);
+    // plain FP32 141 vs 134 (id 0).
+    constexpr int big = TR::SPLIT ? 11 : 6;
+    if (tiles(t[big]) >= cus) return big;
   }
   if (tiles(t[0]) >= cus) return 0;
   return 1;
@@ -560,8 +975,12 @@ static int run_packed(const TileShape &ts, int epi, int64_t M, int64_t N, int64_
   void *pa = nullptr, *pb = nullptr;
   CORAL_TRY(workspace(s, WS_PACK_A, packed_bytes<TR>(M, K, ts.bm, ts.bk), &pa));
   CORAL_TRY(workspace(s, WS_PACK_B, packed_bytes<TR>(N, K, ts.bn, ts.bk), &pb));
-  CORAL_TRY(launch_pack<TR>(ts, true, (v4u *)pa, M, K, fa, rexp, 1, 0, 0, s));
-  CORAL_TRY(launch_pack<TR>(ts, false, (v4u *)pb, N, K, fb, cexp, 1, 0, 0, s));
+  if constexpr (std::is_same<FA, FetchReal<typename TR::T>>::value) {
+    CORAL_TRY(launch_pack_real<TR>(ts, (v4u *)pa, (v4u *)pb, M, N, K, fa, fb, (uint32_t *)rexp, (uint32_t *)cexp, s));
+  } else {
+    CORAL_TRY(launch_pack<TR>(ts, true, (v4u *)pa, M, K, fa, rexp, 1, 0, 0, s));
+    CORAL_TRY(launch_pack<TR>(ts, false, (v4u *)pb, N, K, fb, cexp, 1, 0, 0, s));
+  }
   GemmArgs g{};
   g.Ap = (const v4u *)pa;
   g.Bp = (const v4u *)pb;
@@ -640,8 +1059,7 @@ static int real_gemm(const char *routine, int32_t op_a, int32_t op_b, int64_t m,
     uint32_t *ex = nullptr;
     CORAL_TRY(workspace(s, WS_AUX0, sizeof(uint32_t) * (size_t)(m + n), (void **)&ex));
     CORAL_TRY(zero_u32(ex, m + n, s));
-    CORAL_TRY(launch_row_max(m, k, fa, ex, 1, 0, fa.sk == 1, s));
-    CORAL_TRY(launch_row_max(n, k, fb, ex + m, 1, 0, fb.sk == 1, s));
+    CORAL_TRY(launch_absmax_real(fa, fb, m, n, k, ex, s));
     return run_packed<TR>(ts, EPI_REAL, m, n, k, fa, fb, ex, ex + m, c, ldc, (double)alpha, 0.0, (double)beta,
                           0.0, tile, s);
   } else {
@@ -674,9 +1092,10 @@ static int complex_gemm_4m(const char *routine, int32_t op_a, int32_t op_b, int6
                         (double)alpha[1], (double)beta[0], (double)beta[1], tile, s);
 }
 
-static int cgemm_3m(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, const float *alpha,
-                    const float *a, int64_t lda, const float *b, int64_t ldb, const float *beta, float *c,
-                    int64_t ldc, coral_stream_t stream) {
+template <class TR>
+static int cgemm_3m(const char *routine, int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k,
+                    const float *alpha, const float *a, int64_t lda, const float *b, int64_t ldb, const float *beta,
+                    float *c, int64_t ldc, coral_stream_t stream) {
   if (!alpha) return -6;
   if (!beta) return -11;
   bool trivial = (alpha[0] == 0.f && alpha[1] == 0.f) || k == 0;
@@ -685,9 +1104,9 @@ static int cgemm_3m(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k,
   CORAL_TRY(ensure_init());
   hipStream_t s = resolve(stream);
   if (trivial) return quick_scale<float, true>(c, m, n, ldc, beta[0], beta[1], s);
-  int tile = tuned_tile("cgemm_3m_bf16x3", op_a, op_b, m, n, k);
-  if (tile < 0 || tile >= Menu<TBF3>::n) tile = default_tile<TBF3>(m, n, k);
-  const TileShape ts = Menu<TBF3>::t()[tile];
+  int tile = tuned_tile(routine, op_a, op_b, m, n, k);
+  if (tile < 0 || tile >= Menu<TR>::n) tile = default_tile<TR>(m, n, k);
+  const TileShape ts = Menu<TR>::t()[tile];
   FetchCplx3M<float> fa{a, 0, 0, op_a == CORAL_CONJUGATE_TRANSPOSE};
   FetchCplx3M<float> fb{b, 0, 0, op_b == CORAL_CONJUGATE_TRANSPOSE};
   strides_a<float>(op_a, lda, fa.sr, fa.sk);
@@ -699,13 +1118,13 @@ static int cgemm_3m(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k,
   CORAL_TRY(zero_u32(ex, 3 * (m + n), s));
   CORAL_TRY(launch_row_max(m, k, fa, exa, 3, m, fa.sk == 1, s));
   CORAL_TRY(launch_row_max(n, k, fb, exb, 3, n, fb.sk == 1, s));
-  size_t abytes = packed_bytes<TBF3>(m, k, ts.bm, ts.bk), bbytes = packed_bytes<TBF3>(n, k, ts.bn, ts.bk);
+  size_t abytes = packed_bytes<TR>(m, k, ts.bm, ts.bk), bbytes = packed_bytes<TR>(n, k, ts.bn, ts.bk);
   void *pa = nullptr, *pb = nullptr, *tz = nullptr;
   CORAL_TRY(workspace(s, WS_PACK_A, 3 * abytes, &pa));
   CORAL_TRY(workspace(s, WS_PACK_B, 3 * bbytes, &pb));
   CORAL_TRY(workspace(s, WS_AUX1, 3 * sizeof(float) * (size_t)m * (size_t)n, &tz));
-  CORAL_TRY(launch_pack<TBF3>(ts, true, (v4u *)pa, m, k, fa, exa, 3, (int64_t)(abytes / 16), m, s));
-  CORAL_TRY(launch_pack<TBF3>(ts, false, (v4u *)pb, n, k, fb, exb, 3, (int64_t)(bbytes / 16), n, s));
+  CORAL_TRY(launch_pack<TR>(ts, true, (v4u *)pa, m, k, fa, exa, 3, (int64_t)(abytes / 16), m, s));
+  CORAL_TRY(launch_pack<TR>(ts, false, (v4u *)pb, n, k, fb, exb, 3, (int64_t)(bbytes / 16), n, s));
   GemmArgs g{};
   g.Ap = (const v4u *)pa;
   g.Bp = (const v4u *)pb;
@@ -725,7 +1144,7 @@ static int cgemm_3m(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k,
   g.col_exp = exb;
   g.re_zstride = m;
   g.ce_zstride = n;
-  CORAL_TRY((launch_tile<TBF3, EPI_STORE_Z>(tile, g, 3, s)));
+  CORAL_TRY((launch_tile<TR, EPI_STORE_Z>(tile, g, 3, s)));
   int64_t blocks = std::min<int64_t>(cdiv(m * n, 256), 8192);
   hipLaunchKernelGGL(combine_3m_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (const float *)tz, m, n, c, ldc,
                      alpha[0], alpha[1], beta[0], beta[1], (int)(beta[0] == 0.f && beta[1] == 0.f));
@@ -739,6 +1158,7 @@ int gemm_menu_size(const char *routine) {
   if (!strcmp(routine, "dgemm") || !strcmp(routine, "zgemm")) return gemm::Menu<gemm::TF64>::n;
   if (!strcmp(routine, "sgemm") || !strcmp(routine, "cgemm")) return gemm::Menu<gemm::TF32>::n;
   if (!strcmp(routine, "sgemm_bf16x3") || !strcmp(routine, "cgemm_3m_bf16x3")) return gemm::Menu<gemm::TBF3>::n;
+  if (!strcmp(routine, "sgemm_fp16x3") || !strcmp(routine, "cgemm_3m_fp16x3")) return gemm::Menu<gemm::TF16X3>::n;
   return 0;
 }
 
@@ -776,7 +1196,17 @@ int coral_zgemm(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, con
 int coral_cgemm_3m_bf16x3(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, const float alpha[2],
                           const float *a, int64_t lda, const float *b, int64_t ldb, const float beta[2], float *c,
                           int64_t ldc, coral_stream_t stream) {
-  return cgemm_3m(op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+  return cgemm_3m<TBF3>("cgemm_3m_bf16x3", op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+}
+int coral_sgemm_fp16x3(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, float alpha, const float *a,
+                       int64_t lda, const float *b, int64_t ldb, float beta, float *c, int64_t ldc,
+                       coral_stream_t stream) {
+  return real_gemm<TF16X3, float>("sgemm_fp16x3", op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+}
+int coral_cgemm_3m_fp16x3(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, const float alpha[2],
+                          const float *a, int64_t lda, const float *b, int64_t ldb, const float beta[2], float *c,
+                          int64_t ldc, coral_stream_t stream) {
+  return cgemm_3m<TF16X3>("cgemm_3m_fp16x3", op_a, op_b, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
 }
 
 }  // extern "C"

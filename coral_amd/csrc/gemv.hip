@@ -16,29 +16,35 @@
 #include "coral_common.hpp"
 
 #include <algorithm>
+#include <stdio.h>
+#include <stdlib.h>
 
 namespace coral {
 namespace l2 {
 
-struct alignas(16) Pack32 { uint4 a, b; };
+typedef uint32_t gv4u __attribute__((ext_vector_type(4)));
 
 template <typename E, int RPT>
 __device__ __forceinline__ void load_run(E (&v)[RPT], const E *p, bool vec, int64_t avail) {
+  static_assert((sizeof(E) * RPT) % 16 == 0, "runs are whole 16-byte vectors");
+  constexpr int NV = sizeof(E) * RPT / 16;
   if (vec) {
-    const uint4 *q = reinterpret_cast<const uint4 *>(p);
-    Pack32 pk{q[0], q[1]};
-    __builtin_memcpy(&v[0], &pk, sizeof(E) * RPT);
+    const gv4u *q = reinterpret_cast<const gv4u *>(p);
+    gv4u pk[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) pk[i] = q[i];
+    __builtin_memcpy(&v[0], &pk[0], sizeof(E) * RPT);
   } else {
 #pragma unroll
     for (int r = 0; r < RPT; ++r) v[r] = (r < avail) ? p[r] : ezero<E>();
   }
 }
 
-template <typename E>
+template <typename E, int BYTES>
 __global__ __launch_bounds__(256) void gemv_n_kernel(int64_t m, int64_t n, const E *__restrict__ A, int64_t lda,
                                                      const E *__restrict__ x, int64_t incx, E *__restrict__ P,
                                                      int64_t cpc, int vec_ok) {
-  constexpr int RPT = 32 / sizeof(E);
+  constexpr int RPT = BYTES / sizeof(E);
   const int64_t row0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * RPT;
   if (row0 >= m) return;
   const int64_t c0 = (int64_t)blockIdx.y * cpc;
@@ -106,16 +112,37 @@ __global__ __launch_bounds__(256) void gemv_t_kernel(int64_t m, int64_t n, const
   if (lane == 0) P[(int64_t)blockIdx.y * n + j] = acc;
 }
 
-// y[i] := alpha * sum_c P[c][i] + beta * y[i]   (chunks added in order)
+// y[i] := alpha * sum_c P[c][i] + beta * y[i].  A workgroup takes 64 rows; its four
+// wavefronts sum the chunk ranges [g*q, (g+1)*q) in order (eight loads in flight),
+// and the four group sums are added in group order: a fixed order, so the result
+// does not depend on scheduling.
 template <typename E>
-__global__ void gemv_reduce_kernel(int64_t len, int64_t nchunks, const E *__restrict__ P, E alpha, E beta,
-                                   int beta_zero, E *y, int64_t incy) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= len) return;
+__global__ __launch_bounds__(256) void gemv_reduce_kernel(int64_t len, int64_t nchunks, const E *__restrict__ P,
+                                                          E alpha, E beta, int beta_zero, E *y, int64_t incy) {
+  __shared__ E part[4][64];
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t q = (nchunks + 3) / 4;
+  const int64_t c0 = g * q, c1 = c0 + q < nchunks ? c0 + q : nchunks;
   E s = ezero<E>();
-  for (int64_t c = 0; c < nchunks; ++c) s = eadd(s, P[c * len + i]);
-  E *yp = y + i * incy;
-  *yp = axpby(alpha, s, beta, yp, beta_zero);
+  if (i < len) {
+    int64_t c = c0;
+    for (; c + 8 <= c1; c += 8) {
+      E v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = P[(c + u) * len + i];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s = eadd(s, v[u]);
+    }
+    for (; c < c1; ++c) s = eadd(s, P[c * len + i]);
+  }
+  part[g][lane] = s;
+  __syncthreads();
+  if (g == 0 && i < len) {
+    const E t = eadd(eadd(part[0][lane], part[1][lane]), eadd(part[2][lane], part[3][lane]));
+    E *yp = y + i * incy;
+    *yp = axpby(alpha, t, beta, yp, beta_zero);
+  }
 }
 
 // y := beta*y (alpha == 0 path); beta == 0 stores zeros without reading y.
@@ -175,19 +202,32 @@ int gemv_impl(int32_t trans, int64_t m, int64_t n, E alpha, const E *A, int64_t 
   const int cus = num_cus();
   const bool a_aligned = ((uintptr_t)A % 16 == 0) && ((lda * (int64_t)sizeof(E)) % 16 == 0);
   if (notrans) {
-    const int64_t row_blocks = cdiv(m, 256 * RPT);
-    // enough column chunks for ~8 workgroups per CU, each chunk >= 32 columns
-    int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(cdiv(8 * cus, row_blocks), cdiv(n, 32)));
+    // bytes of each column a thread streams (32: two 16-byte loads, 64: four) and the
+    // workgroup-count target per CU; CORAL_GEMV_CFG=<bytes>,<wg_per_cu> overrides.
+    // measured on 16384^2 f64: 32 bytes x 4 workgroups/CU 5.69 TB/s, x 8 5.18, x 16 5.02;
+    // 64 bytes x 4 5.28 (fewer column chunks also shrink the partial-sum traffic)
+    int cfg_bytes = 32, cfg_wg = 4;
+    if (const char *e = getenv("CORAL_GEMV_CFG")) sscanf(e, "%d,%d", &cfg_bytes, &cfg_wg);
+    if (cfg_bytes != 64) cfg_bytes = 32;
+    if (cfg_wg < 1 || cfg_wg > 64) cfg_wg = 4;
+    const int rpt = cfg_bytes / (int)sizeof(E);
+    const int64_t row_blocks = cdiv(m, 256 * rpt);
+    // enough column chunks for cfg_wg workgroups per CU, each chunk >= 32 columns
+    int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(cdiv((int64_t)cfg_wg * cus, row_blocks), cdiv(n, 32)));
     const int64_t cpc = cdiv(n, chunks);
     chunks = cdiv(n, cpc);
     void *P = nullptr;
     CORAL_TRY(workspace(s, WS_AUX2, sizeof(E) * (size_t)chunks * (size_t)m, &P));
     const int slot = timing_begin(TK_GEMV, s);
-    hipLaunchKernelGGL(gemv_n_kernel<E>, dim3((unsigned)row_blocks, (unsigned)chunks), dim3(256), 0, s, m, n, A, lda,
-                       x, incx, (E *)P, cpc, (int)a_aligned);
+    if (cfg_bytes == 64)
+      hipLaunchKernelGGL((gemv_n_kernel<E, 64>), dim3((unsigned)row_blocks, (unsigned)chunks), dim3(256), 0, s, m, n,
+                         A, lda, x, incx, (E *)P, cpc, (int)a_aligned);
+    else
+      hipLaunchKernelGGL((gemv_n_kernel<E, 32>), dim3((unsigned)row_blocks, (unsigned)chunks), dim3(256), 0, s, m, n,
+                         A, lda, x, incx, (E *)P, cpc, (int)a_aligned);
     CORAL_LAUNCH_CHECK();
     timing_end(slot, s);
-    hipLaunchKernelGGL(gemv_reduce_kernel<E>, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, s, m, chunks,
+    hipLaunchKernelGGL(gemv_reduce_kernel<E>, dim3((unsigned)cdiv(m, 64)), dim3(256), 0, s, m, chunks,
                        (const E *)P, alpha, beta, (int)is_zero_h(beta), y, incy);
     CORAL_LAUNCH_CHECK();
   } else {
@@ -208,7 +248,7 @@ int gemv_impl(int32_t trans, int64_t m, int64_t n, E alpha, const E *A, int64_t 
                          n, A, lda, x, incx, (E *)P, rpc, (int)a_aligned, (int)x_aligned);
     CORAL_LAUNCH_CHECK();
     timing_end(slot, s);
-    hipLaunchKernelGGL(gemv_reduce_kernel<E>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, n, chunks,
+    hipLaunchKernelGGL(gemv_reduce_kernel<E>, dim3((unsigned)cdiv(n, 64)), dim3(256), 0, s, n, chunks,
                        (const E *)P, alpha, beta, (int)is_zero_h(beta), y, incy);
     CORAL_LAUNCH_CHECK();
   }

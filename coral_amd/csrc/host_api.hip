@@ -176,6 +176,8 @@ GEMM_HOST(cgemm, float, 8, const float *, const float *)
 GEMM_HOST(zgemm, double, 16, const double *, const double *)
 GEMM_HOST(sgemm_bf16x3, float, 4, float, float)
 GEMM_HOST(cgemm_3m_bf16x3, float, 8, const float *, const float *)
+GEMM_HOST(sgemm_fp16x3, float, 4, float, float)
+GEMM_HOST(cgemm_3m_fp16x3, float, 8, const float *, const float *)
 
 // ---- level 2 ----
 #define GEMV_HOST(NAME, T, ES, S_T)                                                                              \

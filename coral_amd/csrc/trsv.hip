@@ -98,17 +98,51 @@ template <typename E> __device__ __forceinline__ E eneg(E a) {
 }
 template <typename E> __device__ __forceinline__ E esub(E a, E b) { return eadd(a, eneg(b)); }
 
-// Wavefront 0 solves the diagonal block: rhs holds b_r - (off-diagonal sums) for lane r.
-// T holds the strictly lower part of the block; dvals the diagonal of M.
+// Broadcast lane `l` (wavefront-uniform) through the scalar unit.
+__device__ __forceinline__ float rlane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double rlane(double v, int l) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ float2 rlane(float2 v, int l) { return make_float2(rlane(v.x, l), rlane(v.y, l)); }
+__device__ __forceinline__ double2 rlane(double2 v, int l) { return make_double2(rlane(v.x, l), rlane(v.y, l)); }
+
+// a / d from a precomputed r = 1/d: q = a*r corrected by one Newton step
+// (q + (a - q*d)*r), which gives the rounded quotient for real types.
+__device__ __forceinline__ float qdiv(float a, float d, float r) { float q = a * r; return fmaf(fmaf(-q, d, a), r, q); }
+__device__ __forceinline__ double qdiv(double a, double d, double r) { double q = a * r; return fma(fma(-q, d, a), r, q); }
+__device__ __forceinline__ float2 qdiv(float2 a, float2 d, float2 r) { (void)d; return emul(a, r); }
+__device__ __forceinline__ double2 qdiv(double2 a, double2 d, double2 r) { (void)d; return emul(a, r); }
+template <typename E> __device__ __forceinline__ E erecip(E d) {
+  if constexpr (elem_traits<E>::cplx) {
+    auto den = d.x * d.x + d.y * d.y;  // 1/d = conj(d) / (re^2 + im^2)  (ctrlsv.rs:205-214)
+    E r;
+    r.x = d.x / den;
+    r.y = -d.y / den;
+    return r;
+  } else {
+    return E(1) / d;
+  }
+}
+
+// Wavefront 0 solves the diagonal block: rhs holds b_r - (off-diagonal sums) for lane r,
+// T the strictly lower part of the block, dval the diagonal of M for lane r.  Every
+// lane computes its reciprocal up front; the sequential part per column is one
+// corrected multiply, a scalar broadcast and one fused update.
 template <typename E>
 __device__ __forceinline__ E solve_diag_block(E rhs, E (*T)[NB + 1], E dval, int unit, int nrows) {
   const int lane = threadIdx.x & 63;
+  const E rec = unit ? ezero<E>() : erecip(dval);
   E xr = ezero<E>();
   for (int c = 0; c < nrows; ++c) {
-    E xc_own = unit ? rhs : ediv(rhs, dval);
-    E xc = eshfl(xc_own, c);
+    const E xc_own = unit ? rhs : qdiv(rhs, dval, rec);
+    const E xc = rlane(xc_own, c);
     if (lane == c) xr = xc;
-    if (lane > c) rhs = esub(rhs, emul(T[c][lane], xc));
+    if (lane > c) rhs = emad(rhs, eneg(T[c][lane]), xc);
   }
   return xr;
 }
@@ -127,14 +161,16 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
   int *flags = ctrl + 1;
   const int64_t i0 = (int64_t)b * NB;
   E acc = ezero<E>();
-  E reg[16];
-  if (b > 0) load_tile(v, i0, 0, false, reg);
-  for (int j = 0; j < b; ++j) {
-    // wait for x_j
+  // Off-diagonal tiles of this row panel stream through two register slots
+  // (slot j&1 holds tile j, loaded two tiles ahead); the unrolled-by-two loop keeps
+  // the slot indices static.
+  E reg0[16], reg1[16];
+  auto step = [&](int j, E(&cur)[16]) {
     if (t == 0) {
       // Tickets make this wait finite (block j belongs to a workgroup that started
       // earlier and waits only on smaller tickets); the spin bound is a backstop
       // that records an error instead of hanging if that invariant were broken.
+      // The agent-scope acquire makes x_j visible to the workgroup after the barrier.
       uint32_t spins = 0;
       while (__hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
         __builtin_amdgcn_s_sleep(2);
@@ -144,24 +180,30 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
         }
       }
     }
-    __syncthreads();
-    __threadfence();
-    store_tile(v, T, reg);
+    __syncthreads();  // x_j published; every wavefront is done reading T and xs
+    store_tile(v, T, cur);
     if (t < NB) {
       const int64_t c = (int64_t)j * NB + t;
       xs[t] = c < v.n ? x[v.map(c) * incx] : ezero<E>();
     }
     __syncthreads();
-    if (j + 1 < b) load_tile(v, i0, (int64_t)(j + 1) * NB, false, reg);  // prefetch the next tile
+    if (j + 2 < b) load_tile(v, i0, (int64_t)(j + 2) * NB, false, cur);
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int c = q * 16 + k;
       acc = emad(acc, T[c][lane], xs[c]);
     }
-    __syncthreads();
+  };
+  if (b > 0) load_tile(v, i0, 0, false, reg0);
+  if (b > 1) load_tile(v, i0, NB, false, reg1);
+  for (int j = 0; j < b; j += 2) {
+    step(j, reg0);
+    if (j + 1 < b) step(j + 1, reg1);
   }
+  __syncthreads();  // last panel tile consumed before T is reused
   part[q][lane] = acc;
   // diagonal block
+  E reg[16];
   load_tile(v, i0, i0, true, reg);
   store_tile(v, T, reg);
   __syncthreads();

@@ -134,6 +134,16 @@ int coral_sgemm_bf16x3(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t
 int coral_cgemm_3m_bf16x3(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k,
                           const float alpha[2], const float *a, int64_t lda, const float *b, int64_t ldb,
                           const float beta[2], float *c, int64_t ldc, coral_stream_t stream);
+/* The same split with FP16 halves (11 + 11 significand bits, three FP16 MFMAs, FP32
+ * accumulation; rows/columns scaled so their maximum lies in [2^14, 2^15)).  Same
+ * MFMA rate as the BF16 form; the 22-bit operand representation brings the error
+ * to that of plain FP32 MFMA.  No reference counterpart. */
+int coral_sgemm_fp16x3(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k,
+                       float alpha, const float *a, int64_t lda, const float *b, int64_t ldb,
+                       float beta, float *c, int64_t ldc, coral_stream_t stream);
+int coral_cgemm_3m_fp16x3(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k,
+                          const float alpha[2], const float *a, int64_t lda, const float *b, int64_t ldb,
+                          const float beta[2], float *c, int64_t ldc, coral_stream_t stream);
 
 /* Batched uneven GEMM (no reference counterpart; equivalent to a loop of
  * coral-aarch64/src/level3/sgemm.rs:106 / dgemm.rs:107 calls).  All arrays are
@@ -270,6 +280,10 @@ int coralh_zgemm(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, co
 int coralh_sgemm_bf16x3(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, float alpha,
                         const float *a, int64_t lda, const float *b, int64_t ldb, float beta, float *c, int64_t ldc);
 int coralh_cgemm_3m_bf16x3(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, const float alpha[2],
+                           const float *a, int64_t lda, const float *b, int64_t ldb, const float beta[2], float *c, int64_t ldc);
+int coralh_sgemm_fp16x3(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, float alpha,
+                        const float *a, int64_t lda, const float *b, int64_t ldb, float beta, float *c, int64_t ldc);
+int coralh_cgemm_3m_fp16x3(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, const float alpha[2],
                            const float *a, int64_t lda, const float *b, int64_t ldb, const float beta[2], float *c, int64_t ldc);
 int coralh_sgemv(int32_t trans, int64_t m, int64_t n, float alpha, const float *a, int64_t lda,
                  const float *x, int64_t incx, float beta, float *y, int64_t incy);

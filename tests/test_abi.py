@@ -19,7 +19,8 @@ def test_header_declares_the_call_surface():
             assert f"coral_{p}{r}" in names and f"coralh_{p}{r}" in names
         assert f"coral_i{p}amax" in names
     for r in ("ssymv", "dsymv", "chemv", "zhemv", "sdot", "ddot", "cdotu", "cdotc", "zdotu", "zdotc",
-              "snrm2", "dnrm2", "scnrm2", "dznrm2", "sgemm_bf16x3", "cgemm_3m_bf16x3", "sgemm_batched",
+              "snrm2", "dnrm2", "scnrm2", "dznrm2", "sgemm_bf16x3", "cgemm_3m_bf16x3", "sgemm_fp16x3",
+              "cgemm_3m_fp16x3", "sgemm_batched",
               "dgemm_batched", "dtrsv_batched", "autotune_run", "autotune_save", "autotune_load"):
         assert f"coral_{r}" in names, r
 

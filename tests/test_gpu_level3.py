@@ -38,12 +38,14 @@ def rand(rng, n, f):
 def test_kat(gpu_lib):
     kat = json.load(open(os.path.join(GOLD, "kat.json")))
     k = kat["sgemm_2x2"]
-    for fn, f in ((blas.sgemm, np.float32), (blas.dgemm, np.float64), (blas.sgemm_bf16x3, np.float32)):
+    for fn, f in ((blas.sgemm, np.float32), (blas.dgemm, np.float64), (blas.sgemm_bf16x3, np.float32),
+                  (blas.sgemm_fp16x3, np.float32)):
         c = np.array(k["c"], f)
         fn(0, 0, 2, 2, 2, k["alpha"], np.array(k["a"], f), 2, np.array(k["b"], f), 2, k["beta"], c, 2)
         assert np.all(np.abs(c - np.array(k["expect"])) < 1e-6), fn.__name__
     k = kat["cgemm_2x2"]
-    for fn, f in ((blas.cgemm, np.float32), (blas.zgemm, np.float64), (blas.cgemm_3m_bf16x3, np.float32)):
+    for fn, f in ((blas.cgemm, np.float32), (blas.zgemm, np.float64), (blas.cgemm_3m_bf16x3, np.float32),
+                  (blas.cgemm_3m_fp16x3, np.float32)):
         c = np.array(k["c"], f)
         fn(0, 0, 2, 2, 2, k["alpha"], np.array(k["a"], f), 2, np.array(k["b"], f), 2, k["beta"], c, 2)
         assert np.all(np.abs(c - np.array(k["expect"])) < 1e-6), fn.__name__
@@ -100,7 +102,13 @@ def test_normwise_bound_all_ops(gpu_lib, dt):
     assert worst <= 1.0, worst
 
 
-@pytest.mark.parametrize("which", ["sgemm_bf16x3", "cgemm_3m_bf16x3"])
+# representation error of one split operand element: hi + lo keeps 16 (BF16) or
+# 22 (FP16) significand bits, so |x - hi - lo| <= 2^-17 |x| (BF16) / 2^-22 |x| (FP16);
+# the dropped lo*lo term is of the same order.
+SPLIT_EPS = {"bf16x3": 2.0 ** -16, "fp16x3": 2.0 ** -21}
+
+
+@pytest.mark.parametrize("which", ["sgemm_bf16x3", "cgemm_3m_bf16x3", "sgemm_fp16x3", "cgemm_3m_fp16x3"])
 def test_split_precision_bound(gpu_lib, which):
     rng = np.random.default_rng(11)
     cplx = which.startswith("c")
@@ -120,7 +128,7 @@ def test_split_precision_bound(gpu_lib, which):
                 fn(oa, ob, m, n, k, al, a, lda, b, ldb, be, out, ldc)
                 ref = ref64.gemm(oa, ob, m, n, k, al, a, lda, b, ldb, be, c, ldc)
                 A, B = ref64.mat(a, ar, ac, lda), ref64.mat(b, br, bc, ldb)
-                eps_split = 2.0 ** -16 + k * 2.0 ** -24
+                eps_split = SPLIT_EPS[which[-6:]] + k * 2.0 ** -24
                 # 3M adds the Gauss cancellation: Im = T3 - T1 - T2 (three products' errors)
                 mult = 3.0 if cplx else 1.0
                 num = np.linalg.norm(out.astype(ref.dtype) - ref)
@@ -129,9 +137,11 @@ def test_split_precision_bound(gpu_lib, which):
     assert worst <= 1.0, worst
 
 
-def test_split_precision_wide_range_rows(gpu_lib):
+@pytest.mark.parametrize("which,bound", [("sgemm_bf16x3", 2.0 ** -14), ("sgemm_fp16x3", 2.0 ** -17)])
+def test_split_precision_wide_range_rows(gpu_lib, which, bound):
     # rows of A and columns of B scaled by 2^e, e in [-20, 20]: the power-of-two
     # row/column scaling keeps the relative error of every element at split level
+    # (for FP16 halves it also keeps every scaled value inside FP16's range)
     rng = np.random.default_rng(12)
     m = n = k = 256
     a = rng.standard_normal((m, k)).astype(np.float32)
@@ -142,11 +152,28 @@ def test_split_precision_wide_range_rows(gpu_lib):
     b = (b * np.ldexp(1.0, eb)[None, :]).astype(np.float32)
     af, bf = np.asfortranarray(a).reshape(-1, order="F"), np.asfortranarray(b).reshape(-1, order="F")
     out = np.zeros(m * n, np.float32)
-    blas.sgemm_bf16x3(0, 0, m, n, k, 1.0, af, m, bf, k, 0.0, out, m)
+    getattr(blas, which)(0, 0, m, n, k, 1.0, af, m, bf, k, 0.0, out, m)
     ref = a.astype(np.float64) @ b.astype(np.float64)
     scale = np.ldexp(1.0, ea)[:, None] * np.ldexp(1.0, eb)[None, :]
     rel = np.abs(out.reshape(n, m).T - ref) / (scale * np.sqrt(k))
-    assert rel.max() < 2.0 ** -14, rel.max()
+    assert rel.max() < bound, rel.max()
+
+
+def test_fp16x3_error_matches_plain_fp32(gpu_lib):
+    # the FP16 split keeps 22 of FP32's 24 significand bits: its normwise error
+    # against the FP64 product stays at the plain FP32-MFMA level (N(0,1) operands)
+    rng = np.random.default_rng(14)
+    m = n = k = 1024
+    a = rng.standard_normal(m * k).astype(np.float32)
+    b = rng.standard_normal(k * n).astype(np.float32)
+    ref = ref64.gemm(0, 0, m, n, k, 1.0, a, m, b, k, 0.0, np.zeros(m * n, np.float32), m)
+    errs = {}
+    for name in ("sgemm", "sgemm_fp16x3", "sgemm_bf16x3"):
+        out = np.zeros(m * n, np.float32)
+        getattr(blas, name)(0, 0, m, n, k, 1.0, a, m, b, k, 0.0, out, m)
+        errs[name] = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    assert errs["sgemm_fp16x3"] <= 1.5 * errs["sgemm"], errs
+    assert errs["sgemm_bf16x3"] > errs["sgemm_fp16x3"], errs
 
 
 @pytest.mark.parametrize("dt", ["s", "d", "c", "z"])
