@@ -9,10 +9,18 @@ runs on the same operands beside it, and the error of both against an FP64 oracl
 is measured on a 256-row sample.  DGEMV N (m = n = 16384, alpha = 1, beta = 0) is
 timed as well.
 
+The FP16x3 split form (FP16 halves, same MFMA rate, 22-bit operands) is timed on
+the same operands and reported beside it with its error.
+
 `roofline` describes the dominant kernel (the BF16x3 MFMA GEMM kernel): its
 duration comes from HIP events the library records around that launch on its
 launch stream (coral_timing_*), its work is 2n^3 FP32-equivalent flops per launch
 and its peak is the BF16 dense MFMA peak over the three products (2500/3 TFLOP/s).
+`traffic` is the HBM bytes per launch of that kernel from the committed rocprofv3
+PMC summary (profiles/gemm_bf16x3_8192_traffic.md), when it is present.
+
+The default warmup is long (20 steps, about 60 ms of MFMA work) because the clock
+settles over the first tens of milliseconds of sustained MFMA load.
 
 `cpu_baseline` times the oracle's restatement of CORAL's CPU SGEMM (the Goto
 5-loop with CORAL's blocking, oracle/coral_port.c) on one host core on a bounded
@@ -33,6 +41,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "SGEMM 8192^3 TFLOP/s BF16x3 vs FP32 MFMA at equal FP64-oracle error; GEMV GB/s"
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "gemm_bf16x3_8192_traffic.md")
 PEAK_BF16 = 2500.0  # dense BF16 MFMA TFLOP/s (MI355X)
 PEAK_FP32 = 157.3
 PEAK_HBM = 8000.0  # GB/s
@@ -81,6 +90,19 @@ def sample_error(A, B, C, rows, k):
     rel_fro = scale * np.linalg.norm(E) / (fro_a * fro_b)
     return {"normwise_bound_ratio": float(normwise), "fro_rel_to_AB": float(rel_fro),
             "max_abs_over_max_ref": float(np.abs(E).max() / np.abs(ref).max())}
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the BF16x3 GEMM kernel from the committed PMC summary
+    (a line `traffic_bytes_per_launch: N`), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(n=3072, runs=3):
@@ -419,8 +441,8 @@ def run_suite(which):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--gemv-n", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -449,6 +471,7 @@ def main():
     B = device.DeviceArray(n * n, np.float32)
     C = device.DeviceArray(n * n, np.float32)
     C2 = device.DeviceArray(n * n, np.float32)
+    C3 = device.DeviceArray(n * n, np.float32)
     device.fill_random(A, synthetic.seed_for(3, "A"), "normal")
     device.fill_random(B, synthetic.seed_for(3, "B"), "normal")
 
@@ -457,6 +480,9 @@ def main():
 
     def step_plain():
         call("coral_sgemm", 0, 0, n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.0, C2.ptr, n, None)
+
+    def step_fp16x3():
+        call("coral_sgemm_fp16x3", 0, 0, n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.0, C3.ptr, n, None)
 
     for _ in range(args.warmup):
         step_bf16x3()
@@ -475,17 +501,22 @@ def main():
     ms_plain = timed_steps(step_plain, plain_steps, 1)
     plain = flops / ms_plain / 1e9
     log(f"plain fp32 sgemm {n}^3: {ms_plain:.3f} ms -> {plain:.1f} TFLOP/s")
+    ms_f16 = timed_steps(step_fp16x3, max(3, args.steps // 2), 2)
+    fp16x3 = flops / ms_f16 / 1e9
+    log(f"fp16x3 sgemm {n}^3: {ms_f16:.3f} ms -> {fp16x3:.1f} TFLOP/s")
 
     # errors against FP64 on a row sample
     Ah = A.to_numpy().reshape(n, n).T
     Bh = B.to_numpy().reshape(n, n).T
     Cb = C.to_numpy().reshape(n, n).T
     Cp = C2.to_numpy().reshape(n, n).T
+    Cf = C3.to_numpy().reshape(n, n).T
     rows = np.random.default_rng(1).choice(n, 256, replace=False)
     err_b = sample_error(Ah, Bh, Cb, rows, n)
     err_p = sample_error(Ah, Bh, Cp, rows, n)
-    log(f"error bf16x3 {err_b}  plain {err_p}")
-    del Ah, Bh, Cb, Cp, A, B, C, C2
+    err_f = sample_error(Ah, Bh, Cf, rows, n)
+    log(f"error bf16x3 {err_b}  plain {err_p}  fp16x3 {err_f}")
+    del Ah, Bh, Cb, Cp, Cf, A, B, C, C2, C3
 
     # DGEMV N
     gm = args.gemv_n
@@ -514,6 +545,7 @@ def main():
 
     achieved = flops / k_ms / 1e9
     peak = PEAK_BF16 / 3.0
+    traffic = profiled_traffic()
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -534,13 +566,17 @@ def main():
         "plain_fp32_tflops": round(plain, 2),
         "plain_fp32_frac_of_fp32_peak": round(plain / PEAK_FP32, 4),
         "bf16x3_over_plain": round(value / plain, 3),
-        "error_vs_fp64": {"bf16x3": err_b, "plain_fp32": err_p},
+        "error_vs_fp64": {"bf16x3": err_b, "plain_fp32": err_p, "fp16x3": err_f},
+        "fp16x3": {"tflops": round(fp16x3, 2), "over_plain": round(fp16x3 / plain, 3),
+                   "note": "same split with FP16 halves (coral_sgemm_fp16x3): 22-bit operands, error at the "
+                           "plain FP32-MFMA level"},
         "dgemv": {"gbps": round(gemv_gbps, 1), "kernel_gbps": round(gemv_kernel_gbps, 1),
                   "frac_of_hbm": round(gemv_kernel_gbps / PEAK_HBM, 4), "us_per_call": round(gms * 1e3, 2)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": round(peak, 2), "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
-                     "kernel": "gemm_kernel<TBF3> (BF16x3 MFMA); work = 2n^3 FP32-equivalent flops per launch; "
-                               "peak = 2500 TFLOP/s BF16 / 3 products",
+                     "frac": round(achieved / peak, 4), "traffic": traffic,
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
+                     "kernel": "gemm_dma_kernel<TBF3, 256x256x32, 8 waves> (BF16x3 MFMA); work = 2n^3 "
+                               "FP32-equivalent flops per launch; peak = 2500 TFLOP/s BF16 / 3 products",
                      "kernel_ms": round(k_ms, 4)},
     }
     if not args.no_cpu_baseline:

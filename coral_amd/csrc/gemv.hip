@@ -5,14 +5,14 @@
 // (dgemv_notranspose.rs:88) and x by alpha (:93).  Here both variants are one pass of
 // HBM streaming over A with deterministic two-level reductions:
 //
-//  N: a workgroup owns 256*RPT consecutive rows x a chunk of columns; each thread
-//     keeps RPT row sums and reads 32 contiguous bytes of every column it visits
-//     (two 16-byte loads; four columns in flight).  Column chunks write partial
-//     row sums, which a second kernel adds in chunk order and combines with
-//     alpha/beta.
-//  T/C: one wavefront per column; lanes walk the column 32 bytes at a time and the
-//     dot product is finished with a DPP/shuffle butterfly.  Long columns are
+//  N: a workgroup owns 256*NV*EV rows x a chunk of columns (EV elements per 16-byte
+//     vector); each thread keeps NV*EV row sums and reads NV vectors of every
+//     column it visits, four columns in flight.  Column chunks write partial row
+//     sums, which a second kernel adds in chunk order and combines with alpha/beta.
+//  T/C: one wavefront per column; lanes walk the column two vectors at a time and
+//     the dot product is finished with a DPP/shuffle butterfly.  Long columns are
 //     split into row chunks whose partial dots are reduced the same way.
+//  A is streamed once, with nontemporal loads.
 #include "coral_common.hpp"
 
 #include <algorithm>
@@ -24,69 +24,91 @@ namespace l2 {
 
 typedef uint32_t gv4u __attribute__((ext_vector_type(4)));
 
-template <typename E, int RPT>
-__device__ __forceinline__ void load_run(E (&v)[RPT], const E *p, bool vec, int64_t avail) {
-  static_assert((sizeof(E) * RPT) % 16 == 0, "runs are whole 16-byte vectors");
-  constexpr int NV = sizeof(E) * RPT / 16;
-  if (vec) {
-    const gv4u *q = reinterpret_cast<const gv4u *>(p);
-    gv4u pk[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) pk[i] = q[i];
-    __builtin_memcpy(&v[0], &pk[0], sizeof(E) * RPT);
+// One 16-byte vector (EV = 16 / sizeof(E) elements) at p: a nontemporal load when
+// the whole vector is in range and aligned, otherwise guarded element loads.  The
+// lanes of a wave always cover one contiguous 1 KiB run per load instruction: with
+// nontemporal loads a run split over two instructions is fetched twice.
+template <typename E>
+__device__ __forceinline__ void load_vec(E (&v)[16 / sizeof(E)], const E *p, bool vec, int64_t avail) {
+  constexpr int EV = 16 / sizeof(E);
+  if (vec && avail >= EV) {
+    const gv4u pk = __builtin_nontemporal_load(reinterpret_cast<const gv4u *>(p));
+    __builtin_memcpy(&v[0], &pk, 16);
   } else {
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) v[r] = (r < avail) ? p[r] : ezero<E>();
+    for (int r = 0; r < EV; ++r) v[r] = (r < avail) ? p[r] : ezero<E>();
   }
 }
 
-template <typename E, int BYTES>
+// N: a workgroup owns 256 * NV * EV rows; lane l of wave w holds rows
+// base_w + v*64*EV + l*EV + e (v < NV, e < EV) and reads them from every column of
+// its chunk, four columns in flight.
+template <typename E, int NV>
 __global__ __launch_bounds__(256) void gemv_n_kernel(int64_t m, int64_t n, const E *__restrict__ A, int64_t lda,
                                                      const E *__restrict__ x, int64_t incx, E *__restrict__ P,
                                                      int64_t cpc, int vec_ok) {
-  constexpr int RPT = BYTES / sizeof(E);
-  const int64_t row0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * RPT;
-  if (row0 >= m) return;
+  constexpr int EV = 16 / sizeof(E);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t wbase = (int64_t)blockIdx.x * (256 * NV * EV) + (int64_t)w * (64 * NV * EV);
+  if (wbase >= m) return;  // uniform per wave; no barrier below
   const int64_t c0 = (int64_t)blockIdx.y * cpc;
   const int64_t c1 = c0 + cpc < n ? c0 + cpc : n;
-  const bool full = vec_ok && row0 + RPT <= m;
-  const int64_t avail = m - row0;
-  E acc[RPT];
+  int64_t row[NV], avail[NV];
 #pragma unroll
-  for (int r = 0; r < RPT; ++r) acc[r] = ezero<E>();
-  const E *base = A + row0;
+  for (int v = 0; v < NV; ++v) {
+    row[v] = wbase + v * 64 * EV + lane * EV;
+    avail[v] = m - row[v];
+  }
+  E acc[NV][EV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int e = 0; e < EV; ++e) acc[v][e] = ezero<E>();
+  const bool vec = vec_ok;
   int64_t j = c0;
   for (; j + 4 <= c1; j += 4) {
-    E a[4][RPT];
+    E a[4][NV][EV];
     E xv[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       xv[u] = x[(j + u) * incx];
-      load_run<E, RPT>(a[u], base + (j + u) * lda, full, avail);
+      const E *col = A + (j + u) * lda;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) load_vec<E>(a[u][v], col + row[v], vec, avail[v]);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int r = 0; r < RPT; ++r) acc[r] = emad(acc[r], a[u][r], xv[u]);
+      for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int e = 0; e < EV; ++e) acc[v][e] = emad(acc[v][e], a[u][v][e], xv[u]);
   }
   for (; j < c1; ++j) {
-    E a[RPT];
-    E xv = x[j * incx];
-    load_run<E, RPT>(a, base + j * lda, full, avail);
+    const E xv = x[j * incx];
+    const E *col = A + j * lda;
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) acc[r] = emad(acc[r], a[r], xv);
+    for (int v = 0; v < NV; ++v) {
+      E a[EV];
+      load_vec<E>(a, col + row[v], vec, avail[v]);
+#pragma unroll
+      for (int e = 0; e < EV; ++e) acc[v][e] = emad(acc[v][e], a[e], xv);
+    }
   }
-  E *out = P + (int64_t)blockIdx.y * m + row0;
+  E *out = P + (int64_t)blockIdx.y * m;
 #pragma unroll
-  for (int r = 0; r < RPT; ++r)
-    if (r < avail) out[r] = acc[r];
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int e = 0; e < EV; ++e)
+      if (e < avail[v]) out[row[v] + e] = acc[v][e];
 }
 
+// T/C: one wavefront per column; per step lane l reads vectors at
+// i + v*64*EV + l*EV (v < 2) of the column and of x.
 template <typename E, bool CONJ>
 __global__ __launch_bounds__(256) void gemv_t_kernel(int64_t m, int64_t n, const E *__restrict__ A, int64_t lda,
                                                      const E *__restrict__ x, int64_t incx, E *__restrict__ P,
                                                      int64_t rpc, int vec_ok, int xvec_ok) {
-  constexpr int RPT = 32 / sizeof(E);
+  constexpr int EV = 16 / sizeof(E), NV = 2;
   const int lane = threadIdx.x & 63;
   const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (j >= n) return;  // uniform per wavefront; no block-level barrier below
@@ -94,19 +116,22 @@ __global__ __launch_bounds__(256) void gemv_t_kernel(int64_t m, int64_t n, const
   const int64_t r1 = r0 + rpc < m ? r0 + rpc : m;
   const E *col = A + j * lda;
   E acc = ezero<E>();
-  for (int64_t i = r0 + lane * RPT; i < r1; i += 64 * RPT) {
-    const int64_t avail = r1 - i;
-    const bool full = avail >= RPT;
-    E a[RPT], xv[RPT];
-    load_run<E, RPT>(a, col + i, full && vec_ok, avail);
-    if (incx == 1) {
-      load_run<E, RPT>(xv, x + i, full && xvec_ok, avail);
-    } else {
+  for (int64_t i = r0; i < r1; i += 64 * EV * NV) {
 #pragma unroll
-      for (int r = 0; r < RPT; ++r) xv[r] = (r < avail) ? x[(i + r) * incx] : ezero<E>();
+    for (int v = 0; v < NV; ++v) {
+      const int64_t rr = i + v * 64 * EV + lane * EV;
+      const int64_t avail = r1 - rr;
+      E a[EV], xv[EV];
+      load_vec<E>(a, col + rr, vec_ok, avail);
+      if (incx == 1) {
+        load_vec<E>(xv, x + rr, xvec_ok, avail);
+      } else {
+#pragma unroll
+        for (int e = 0; e < EV; ++e) xv[e] = (e < avail) ? x[(rr + e) * incx] : ezero<E>();
+      }
+#pragma unroll
+      for (int e = 0; e < EV; ++e) acc = emad<CONJ>(acc, a[e], xv[e]);
     }
-#pragma unroll
-    for (int r = 0; r < RPT; ++r) acc = emad<CONJ>(acc, a[r], xv[r]);
   }
   acc = wave_esum(acc);
   if (lane == 0) P[(int64_t)blockIdx.y * n + j] = acc;
@@ -198,20 +223,19 @@ int gemv_impl(int32_t trans, int64_t m, int64_t n, E alpha, const E *A, int64_t 
   if (!A) return -5;
   if (!x) return -7;
   (void)xlen;
-  constexpr int RPT = 32 / sizeof(E);
   const int cus = num_cus();
   const bool a_aligned = ((uintptr_t)A % 16 == 0) && ((lda * (int64_t)sizeof(E)) % 16 == 0);
   if (notrans) {
-    // bytes of each column a thread streams (32: two 16-byte loads, 64: four) and the
-    // workgroup-count target per CU; CORAL_GEMV_CFG=<bytes>,<wg_per_cu> overrides.
-    // measured on 16384^2 f64: 32 bytes x 4 workgroups/CU 5.69 TB/s, x 8 5.18, x 16 5.02;
-    // 64 bytes x 4 5.28 (fewer column chunks also shrink the partial-sum traffic)
-    int cfg_bytes = 32, cfg_wg = 4;
-    if (const char *e = getenv("CORAL_GEMV_CFG")) sscanf(e, "%d,%d", &cfg_bytes, &cfg_wg);
-    if (cfg_bytes != 64) cfg_bytes = 32;
-    if (cfg_wg < 1 || cfg_wg > 64) cfg_wg = 4;
-    const int rpt = cfg_bytes / (int)sizeof(E);
-    const int64_t row_blocks = cdiv(m, 256 * rpt);
+    // 16-byte vectors per column a thread streams (2 or 4) and the workgroup-count
+    // target per CU; CORAL_GEMV_CFG=<vectors>,<wg_per_cu> overrides.  Measured on
+    // 16384^2 f64 with nontemporal loads: 2 vectors x 1 workgroup/CU 6.39 TB/s,
+    // x 2 6.18, x 4 6.04 (more column chunks also add partial-sum traffic).
+    int cfg_nv = 2, cfg_wg = 1;
+    if (const char *e = getenv("CORAL_GEMV_CFG")) sscanf(e, "%d,%d", &cfg_nv, &cfg_wg);
+    if (cfg_nv != 4) cfg_nv = 2;
+    if (cfg_wg < 1 || cfg_wg > 64) cfg_wg = 1;
+    const int64_t rows_per_wg = 256 * cfg_nv * (16 / (int)sizeof(E));
+    const int64_t row_blocks = cdiv(m, rows_per_wg);
     // enough column chunks for cfg_wg workgroups per CU, each chunk >= 32 columns
     int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(cdiv((int64_t)cfg_wg * cus, row_blocks), cdiv(n, 32)));
     const int64_t cpc = cdiv(n, chunks);
@@ -219,11 +243,11 @@ int gemv_impl(int32_t trans, int64_t m, int64_t n, E alpha, const E *A, int64_t 
     void *P = nullptr;
     CORAL_TRY(workspace(s, WS_AUX2, sizeof(E) * (size_t)chunks * (size_t)m, &P));
     const int slot = timing_begin(TK_GEMV, s);
-    if (cfg_bytes == 64)
-      hipLaunchKernelGGL((gemv_n_kernel<E, 64>), dim3((unsigned)row_blocks, (unsigned)chunks), dim3(256), 0, s, m, n,
+    if (cfg_nv == 4)
+      hipLaunchKernelGGL((gemv_n_kernel<E, 4>), dim3((unsigned)row_blocks, (unsigned)chunks), dim3(256), 0, s, m, n,
                          A, lda, x, incx, (E *)P, cpc, (int)a_aligned);
     else
-      hipLaunchKernelGGL((gemv_n_kernel<E, 32>), dim3((unsigned)row_blocks, (unsigned)chunks), dim3(256), 0, s, m, n,
+      hipLaunchKernelGGL((gemv_n_kernel<E, 2>), dim3((unsigned)row_blocks, (unsigned)chunks), dim3(256), 0, s, m, n,
                          A, lda, x, incx, (E *)P, cpc, (int)a_aligned);
     CORAL_LAUNCH_CHECK();
     timing_end(slot, s);
@@ -232,7 +256,7 @@ int gemv_impl(int32_t trans, int64_t m, int64_t n, E alpha, const E *A, int64_t 
     CORAL_LAUNCH_CHECK();
   } else {
     const int64_t col_blocks = cdiv(n, 4);
-    const int64_t unit = 64 * RPT;
+    const int64_t unit = 64 * 2 * (16 / (int64_t)sizeof(E));  // rows per wave step
     int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(cdiv(8 * cus, col_blocks), cdiv(m, 4 * unit)));
     const int64_t rpc = cdiv(cdiv(m, chunks), unit) * unit;
     chunks = cdiv(m, rpc);
