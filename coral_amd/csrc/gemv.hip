@@ -103,12 +103,13 @@ __global__ __launch_bounds__(256) void gemv_n_kernel(int64_t m, int64_t n, const
 }
 
 // T/C: one wavefront per column; per step lane l reads vectors at
-// i + v*64*EV + l*EV (v < 2) of the column and of x.
+// i + v*64*EV + l*EV (v < kTVecs) of the column and of x.
+constexpr int kTVecs = 4;
 template <typename E, bool CONJ>
 __global__ __launch_bounds__(256) void gemv_t_kernel(int64_t m, int64_t n, const E *__restrict__ A, int64_t lda,
                                                      const E *__restrict__ x, int64_t incx, E *__restrict__ P,
                                                      int64_t rpc, int vec_ok, int xvec_ok) {
-  constexpr int EV = 16 / sizeof(E), NV = 2;
+  constexpr int EV = 16 / sizeof(E), NV = kTVecs;
   const int lane = threadIdx.x & 63;
   const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (j >= n) return;  // uniform per wavefront; no block-level barrier below
@@ -256,7 +257,7 @@ int gemv_impl(int32_t trans, int64_t m, int64_t n, E alpha, const E *A, int64_t 
     CORAL_LAUNCH_CHECK();
   } else {
     const int64_t col_blocks = cdiv(n, 4);
-    const int64_t unit = 64 * 2 * (16 / (int64_t)sizeof(E));  // rows per wave step
+    const int64_t unit = 64 * kTVecs * (16 / (int64_t)sizeof(E));  // rows per wave step
     int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(cdiv(8 * cus, col_blocks), cdiv(m, 4 * unit)));
     const int64_t rpc = cdiv(cdiv(m, chunks), unit) * unit;
     chunks = cdiv(m, rpc);

@@ -4,9 +4,9 @@
 // CORAL walks the stored triangle once per column (lda == n path,
 // coral-aarch64/src/level2/chemv.rs:142-231): every stored a_ij feeds y_i (axpy)
 // and y_j (conjugated dot), the diagonal contributes Re(a_jj) only (:178-182), and
-// the other triangle is never touched.  Here the stored triangle is cut into
+// the other triangle is never used.  Here the stored triangle is cut into
 // NB x NB blocks, one workgroup per block, and each block is streamed from HBM
-// exactly once:
+// exactly once (a diagonal block is read whole; its unstored half is masked out):
 //   off-diagonal block (r, c):  u = A_rc * x_c  -> partial for row block r
 //                               v = A_rc^H * x_r -> partial for row block c
 //   diagonal block (b, b):      the stored half of the block, both roles.
@@ -21,7 +21,7 @@
 namespace coral {
 namespace sym {
 
-constexpr int NB = 128;  // block edge; 4 wavefronts x 32 columns, 2 rows per lane
+constexpr int NB = 64;  // block edge
 
 template <typename E> __device__ __forceinline__ E real_part(E a) {
   if constexpr (elem_traits<E>::cplx) { a.y = 0; return a; }
@@ -37,96 +37,146 @@ __device__ __forceinline__ void block_of(int64_t t, int64_t &lo, int64_t &hi) {
   lo = t - c * (c + 1) / 2;
 }
 
+typedef uint32_t sv4u __attribute__((ext_vector_type(4)));
+
+// One stored NB x NB block per workgroup.  The block is streamed into LDS with
+// 16-byte nontemporal loads (each half-wave or wave covers one contiguous column
+// run), then u = A_blk x_c (by rows) and v = A_blk^H x_r (by columns) are formed
+// from LDS by four row/column segments each and combined in segment order.
 template <typename E, bool HERM>
 __global__ __launch_bounds__(256) void symv_block_kernel(int64_t n, int upper, const E *__restrict__ A, int64_t lda,
                                                          const E *__restrict__ x, int64_t incx, E *__restrict__ Q,
-                                                         int64_t nb) {
+                                                         int64_t nb, int vec_ok) {
+  constexpr int EV = 16 / sizeof(E);         // elements per 16-byte vector
+  constexpr int VPC = NB / EV;               // vectors per block column
+  constexpr int LDP = NB + 1;                // padded column length: column reads hit distinct banks
+  __shared__ E s[NB * LDP];                  // s[c * LDP + r]
   __shared__ E xr_s[NB], xc_s[NB];
-  __shared__ E vd_s[4][NB];  // v of diagonal-block columns, by owning wavefront
+  __shared__ E up[4][NB], vp[4][NB];
   int64_t lo, hi;
   block_of(blockIdx.x, lo, hi);
-  // stored block (br, bc): upper -> br <= bc, lower -> br >= bc
-  const int64_t br = upper ? lo : hi, bc = upper ? hi : lo;
+  // stored block (br, bc).  Consecutive workgroups take consecutive row blocks of
+  // one block column (upper: rows 0..hi of column hi; lower: the mirror image), so
+  // concurrently running workgroups read neighbouring runs of the same columns.
+  const int64_t br = upper ? lo : nb - 1 - lo, bc = upper ? hi : nb - 1 - hi;
   const int64_t r0 = br * NB, c0 = bc * NB;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (tid < NB) {
-    int64_t i = r0 + tid;
-    xr_s[tid] = i < n ? x[i * incx] : ezero<E>();
-  } else {
-    int64_t j = c0 + tid - NB;
-    xc_s[tid - NB] = j < n ? x[j * incx] : ezero<E>();
-  }
-  __syncthreads();
+  const int t = threadIdx.x;
   const bool diag = br == bc;
-  const int64_t ra = r0 + lane, rb = r0 + lane + 64;  // the two rows of this lane
-  E ua = ezero<E>(), ub = ezero<E>();
-  E xa = xr_s[lane], xb = xr_s[lane + 64];
-  for (int jj = wave; jj < NB; jj += 4) {
-    const int64_t j = c0 + jj;
-    E v = ezero<E>();
-    if (j < n) {
-      const E *col = A + j * lda;
-      const E xj = xc_s[jj];
-      if (!diag) {
-        E a0 = ra < n ? col[ra] : ezero<E>();
-        E a1 = rb < n ? col[rb] : ezero<E>();
-        ua = emad(ua, a0, xj);
-        ub = emad(ub, a1, xj);
-        v = emad<HERM>(v, a0, xa);
-        v = emad<HERM>(v, a1, xb);
-      } else {
-        // stored half only: upper keeps row <= col, lower keeps row >= col
-        const int64_t rl[2] = {ra, rb};
+  // ---- stream the block into LDS: all loads issued before the first store ----
+  constexpr int IT = NB * VPC / 256;  // vectors per thread
+  const bool full = vec_ok && c0 + NB <= n && r0 + NB <= n;
+  E blk[IT][EV];
+  if (full) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int64_t r = rl[h];
-          if (r >= n) continue;
-          const bool stored = upper ? (r <= j) : (r >= j);
-          if (!stored) continue;
-          E a = col[r];
-          if (r == j) {
-            E ad = HERM ? real_part(a) : a;
-            if (h == 0) ua = emad(ua, ad, xj); else ub = emad(ub, ad, xj);
-          } else {
-            if (h == 0) ua = emad(ua, a, xj); else ub = emad(ub, a, xj);
-            v = emad<HERM>(v, a, h == 0 ? xa : xb);
-          }
-        }
-      }
+    for (int it = 0; it < IT; ++it) {
+      const int q = t + 256 * it;
+      const int c = q / VPC, rv = (q % VPC) * EV;
+      const sv4u pk = __builtin_nontemporal_load(reinterpret_cast<const sv4u *>(A + (c0 + c) * lda + r0 + rv));
+      __builtin_memcpy(&blk[it][0], &pk, 16);
     }
-    v = wave_esum(v);
-    if (lane == 0) {
-      // v is the column-block contribution for row jj of block bc
-      if (!diag) Q[(bc * nb + br) * NB + jj] = v;
-      else vd_s[wave][jj] = v;  // staged; combined with u below (diag block: both into Q[b][b])
+  } else {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int q = t + 256 * it;
+      const int c = q / VPC, rv = (q % VPC) * EV;
+      const int64_t gr = r0 + rv, gc = c0 + c;
+#pragma unroll
+      for (int e = 0; e < EV; ++e) blk[it][e] = (gc < n && gr + e < n) ? A[gc * lda + gr + e] : ezero<E>();
     }
   }
-  // u from the four wavefronts -> partial for row block br
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int q = t + 256 * it;
+    const int c = q / VPC, rv = (q % VPC) * EV;
+#pragma unroll
+    for (int e = 0; e < EV; ++e) s[c * LDP + rv + e] = blk[it][e];
+  }
+  if (t < NB) {
+    const int64_t i = r0 + t;
+    xr_s[t] = i < n ? x[i * incx] : ezero<E>();
+  } else if (t < 2 * NB) {
+    const int64_t j = c0 + t - NB;
+    xc_s[t - NB] = j < n ? x[j * incx] : ezero<E>();
+  }
   __syncthreads();
-  __shared__ E uw[4][NB];
-  uw[wave][lane] = ua;
-  uw[wave][lane + 64] = ub;
-  __syncthreads();
-  if (tid < NB) {
-    E s = eadd(eadd(uw[0][tid], uw[1][tid]), eadd(uw[2][tid], uw[3][tid]));
-    if (diag) {
-      // v of column tid was written by wave (tid % 4)
-      s = eadd(s, vd_s[tid & 3][tid]);
+  // ---- u (rows) and v (columns), four segments of NB/4 each ----
+  constexpr int SEG = NB / 4;
+  const int idx = t % NB, seg = t / NB, k0 = seg * SEG;
+  E xcv[SEG], xrv[SEG];
+#pragma unroll
+  for (int kk = 0; kk < SEG; ++kk) {  // broadcast reads, once
+    xcv[kk] = xc_s[k0 + kk];
+    xrv[kk] = xr_s[k0 + kk];
+  }
+  E u = ezero<E>(), v = ezero<E>();
+  if (!diag) {
+#pragma unroll
+    for (int kk = 0; kk < SEG; ++kk) {
+      u = emad(u, s[(k0 + kk) * LDP + idx], xcv[kk]);        // row idx, column k
+      v = emad<HERM>(v, s[idx * LDP + k0 + kk], xrv[kk]);    // column idx, row k
     }
-    Q[(br * nb + bc) * NB + tid] = s;
+  } else {
+    // stored half only (upper: row <= column); the diagonal counts once, in u,
+    // with its imaginary part dropped for Hermitian matrices
+#pragma unroll
+    for (int kk = 0; kk < SEG; ++kk) {
+      const int k = k0 + kk;
+      if (upper ? idx <= k : idx >= k) {
+        E a = s[k * LDP + idx];
+        if (HERM && idx == k) a = real_part(a);
+        u = emad(u, a, xcv[kk]);
+      }
+      if (upper ? k < idx : k > idx) v = emad<HERM>(v, s[idx * LDP + k], xrv[kk]);
+    }
+  }
+  up[seg][idx] = u;
+  vp[seg][idx] = v;
+  __syncthreads();
+  if (t < NB) {
+    const E uu = eadd(eadd(up[0][t], up[1][t]), eadd(up[2][t], up[3][t]));
+    const E vv = eadd(eadd(vp[0][t], vp[1][t]), eadd(vp[2][t], vp[3][t]));
+    if (diag) {
+      Q[(br * nb + bc) * NB + t] = eadd(uu, vv);
+    } else {
+      Q[(br * nb + bc) * NB + t] = uu;  // partial of row block br from partner bc
+      Q[(bc * nb + br) * NB + t] = vv;  // partial of row block bc from partner br
+    }
   }
 }
 
+// y[i] := alpha * sum_c Q[r][c][w] + beta * y[i] with i = r*NB + w.  A workgroup
+// takes 64 rows; its four wavefronts sum partner ranges [g*q, (g+1)*q) in order
+// (eight loads in flight) and the group sums are added in group order, so the
+// result does not depend on scheduling.
 template <typename E>
-__global__ void symv_reduce_kernel(int64_t n, int64_t nb, const E *__restrict__ Q, E alpha, E beta, int beta_zero,
-                                   E *y, int64_t incy) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ __launch_bounds__(256) void symv_reduce_kernel(int64_t n, int64_t nb, const E *__restrict__ Q, E alpha,
+                                                          E beta, int beta_zero, E *y, int64_t incy) {
+  __shared__ E part[4][64];
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
   const int64_t r = i / NB, w = i % NB;
-  E s = ezero<E>();
-  for (int64_t c = 0; c < nb; ++c) s = eadd(s, Q[(r * nb + c) * NB + w]);
-  E *yp = y + i * incy;
-  *yp = axpby(alpha, s, beta, yp, beta_zero);
+  const int64_t q = (nb + 3) / 4;
+  const int64_t c0 = g * q, c1 = c0 + q < nb ? c0 + q : nb;
+  E acc = ezero<E>();
+  if (i < n) {
+    const E *base = Q + r * nb * NB + w;
+    int64_t c = c0;
+    for (; c + 8 <= c1; c += 8) {
+      E v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = base[(c + u) * NB];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = eadd(acc, v[u]);
+    }
+    for (; c < c1; ++c) acc = eadd(acc, base[c * NB]);
+  }
+  part[g][lane] = acc;
+  __syncthreads();
+  if (g == 0 && i < n) {
+    const E tsum = eadd(eadd(part[0][lane], part[1][lane]), eadd(part[2][lane], part[3][lane]));
+    E *yp = y + i * incy;
+    *yp = axpby(alpha, tsum, beta, yp, beta_zero);
+  }
 }
 
 template <typename E>
@@ -165,10 +215,11 @@ static int symv_impl(int32_t uplo, int64_t n, E alpha, const E *A, int64_t lda, 
   void *Q = nullptr;
   CORAL_TRY(workspace(s, WS_AUX2, sizeof(E) * (size_t)(nb * nb * NB), &Q));
   const int64_t nblocks = nb * (nb + 1) / 2;
+  const int vec_ok = ((uintptr_t)A % 16 == 0) && ((lda * (int64_t)sizeof(E)) % 16 == 0);
   hipLaunchKernelGGL((symv_block_kernel<E, HERM>), dim3((unsigned)nblocks), dim3(256), 0, s, n,
-                     (int)(uplo == CORAL_UPPER_TRIANGULAR), A, lda, x, incx, (E *)Q, nb);
+                     (int)(uplo == CORAL_UPPER_TRIANGULAR), A, lda, x, incx, (E *)Q, nb, vec_ok);
   CORAL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(symv_reduce_kernel<E>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, n, nb, (const E *)Q, alpha,
+  hipLaunchKernelGGL(symv_reduce_kernel<E>, dim3((unsigned)cdiv(n, 64)), dim3(256), 0, s, n, nb, (const E *)Q, alpha,
                      beta, (int)eiszero(beta), y, incy);
   CORAL_LAUNCH_CHECK();
   return CORAL_OK;
