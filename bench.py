@@ -188,10 +188,11 @@ def suite_c2():
     device.fill_random(A, synthetic.seed_for(2, "A"), "normal")
     device.fill_random(B, synthetic.seed_for(2, "B"), "normal")
     for ops in ((0, 0), (1, 1)):
+        step = lambda: call("coral_dgemm", ops[0], ops[1], n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.0, C.ptr, n, None)
+        timed_steps(step, 3, 0)  # warm clocks; kernel timing covers the timed calls only
         call("coral_timing_reset")
         call("coral_timing_enable", 1)
-        ms = timed_steps(lambda: call("coral_dgemm", ops[0], ops[1], n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.0, C.ptr, n,
-                                      None), 3, 1)
+        ms = timed_steps(step, 5, 0)
         k_ms, _ = kernel_avg_ms(0)
         call("coral_timing_enable", 0)
         Ah, Bh, Ch = (_host_mat(x, n, n, np.float64) for x in (A, B, C))
@@ -210,9 +211,11 @@ def suite_c2():
     for d, o in ((GA, "A"), (gx, "x"), (gy, "y")):
         device.fill_random(d, synthetic.seed_for(2, o), "uniform")
     for tr in (0, 1):
+        step = lambda: call("coral_dgemv", tr, m, m, 1.0, GA.ptr, m, gx.ptr, 1, 0.0, gy.ptr, 1, None)
+        timed_steps(step, 5, 0)
         call("coral_timing_reset")
         call("coral_timing_enable", 1)
-        ms = timed_steps(lambda: call("coral_dgemv", tr, m, m, 1.0, GA.ptr, m, gx.ptr, 1, 0.0, gy.ptr, 1, None), 20, 2)
+        ms = timed_steps(step, 20, 0)
         k_ms, _ = kernel_avg_ms(1)
         call("coral_timing_enable", 0)
         by = 8.0 * (m * m + 2 * m)

@@ -740,7 +740,9 @@ template <int BM_, int BN_, int BK_, int WVM_ = 0, int WVN_ = 0, int SCH_ = 0> s
 template <class... S> struct ShapeList {};
 
 // Tile menus per element kind; the index is the tile id the autotuner stores.
-using MenuF64 = ShapeList<Shape<128, 128, 16>, Shape<64, 64, 16>, Shape<128, 128, 32>>;
+using MenuF64 = ShapeList<Shape<128, 128, 16>, Shape<64, 64, 16>, Shape<128, 128, 32>, Shape<128, 128, 16, 2, 2>,
+                          Shape<128, 128, 16, 2, 4>, Shape<64, 64, 32, 2, 2>, Shape<128, 64, 32, 2, 2>,
+                          Shape<256, 128, 16, 2, 4>, Shape<128, 128, 16, 2, 2, 2>, Shape<128, 128, 32, 2, 4, 2>>;
 using MenuF32 = ShapeList<Shape<128, 128, 32>, Shape<64, 64, 32>, Shape<256, 128, 32>, Shape<256, 256, 32>,
                           Shape<128, 128, 64>, Shape<256, 256, 32, 2, 2>, Shape<256, 256, 32, 2, 4>,
                           Shape<128, 128, 64, 2, 2>, Shape<256, 128, 32, 2, 2>, Shape<256, 256, 32, 2, 4, 1>,
@@ -951,8 +953,10 @@ static int default_tile(int64_t M, int64_t N, int64_t K) {
   const int cus = num_cus();
   auto tiles = [&](const TileShape &s) { return cdiv(M, s.bm) * cdiv(N, s.bn); };
   if constexpr (std::is_same<TR, TF64>::value) {
-    // FP64 MFMA is slow per instruction; 64x64 tiles keep 4 waves per SIMD resident
-    // (measured 58.5 vs 47.9 TFLOP/s for 128x128 at 8192^3)
+    // 128x128x16, 8 waves, direct-to-LDS: 70.0 TFLOP/s at 8192^3 (0.89 of the FP64
+    // MFMA peak) vs 61.1 for the register-staged 64x64 tile (id 1), which still
+    // serves grids with fewer 128x128 tiles than CUs
+    if (tiles(t[4]) >= cus) return 4;
     return 1;
   } else {
     // 256x256 tiles, 8 waves, direct-to-LDS loads: fewest LDS bytes per MFMA.  At
