@@ -14,9 +14,10 @@
 //   * workgroups take row blocks of NB = 64 rows from an atomic ticket, so a
 //     workgroup only ever waits on blocks owned by workgroups already running;
 //   * a row block accumulates M(block, j) x_j over earlier blocks j as soon as
-//     block j publishes its flag (tile loads are issued before the wait);
+//     block j's values are published (tile loads are issued before the wait);
 //   * the diagonal block is solved by one wavefront (lane = row), in the
-//     element precision (FP64 for D/Z), and published with a release fence.
+//     element precision (FP64 for D/Z), and published through a sentinel-filled
+//     staging vector whose words are their own ready flags (see Stage below).
 // TRMV (x := op(A) x) uses the same tiles without dependencies: one workgroup per
 // (row block, column block) pair writes a partial, a second kernel sums them.
 #include "coral_common.hpp"
@@ -97,6 +98,9 @@ template <typename E> __device__ __forceinline__ E eneg(E a) {
   if constexpr (elem_traits<E>::cplx) { a.x = -a.x; a.y = -a.y; return a; } else return -a;
 }
 template <typename E> __device__ __forceinline__ E esub(E a, E b) { return eadd(a, eneg(b)); }
+template <typename E> __device__ __forceinline__ E eone() {
+  if constexpr (elem_traits<E>::cplx) { E a{}; a.x = 1; return a; } else return E(1);
+}
 
 // Broadcast lane `l` (wavefront-uniform) through the scalar unit.
 __device__ __forceinline__ float rlane(float v, int l) {
@@ -130,26 +134,76 @@ template <typename E> __device__ __forceinline__ E erecip(E d) {
 }
 
 // Wavefront 0 solves the diagonal block: rhs holds b_r - (off-diagonal sums) for lane r,
-// T the strictly lower part of the block, dval the diagonal of M for lane r.  Every
-// lane computes its reciprocal up front; the sequential part per column is one
-// corrected multiply, a scalar broadcast and one fused update.
-template <typename E>
-__device__ __forceinline__ E solve_diag_block(E rhs, E (*T)[NB + 1], E dval, int unit, int nrows) {
+// col[c] the strictly lower part of column c of the block (lane r holds row r), dval
+// the diagonal of M for lane r.  Every lane computes its reciprocal up front; the
+// sequential part per column is one corrected multiply, a scalar broadcast and one
+// fused update, all in registers (the loop is fully unrolled so the broadcast lane
+// is a constant).
+template <typename E, typename COL>
+__device__ __forceinline__ E solve_diag_block(E rhs, const COL &col, E dval, int unit, int nrows) {
   const int lane = threadIdx.x & 63;
   const E rec = unit ? ezero<E>() : erecip(dval);
   E xr = ezero<E>();
-  for (int c = 0; c < nrows; ++c) {
-    const E xc_own = unit ? rhs : qdiv(rhs, dval, rec);
-    const E xc = rlane(xc_own, c);
-    if (lane == c) xr = xc;
-    if (lane > c) rhs = emad(rhs, eneg(T[c][lane]), xc);
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    if (c < nrows) {
+      const E xc_own = unit ? rhs : qdiv(rhs, dval, rec);
+      const E xc = rlane(xc_own, c);
+      if (lane == c) xr = xc;
+      if (lane > c) rhs = emad(rhs, eneg(col(c)), xc);
+    }
   }
   return xr;
 }
 
+// Published solution blocks.  Every solved x value is also written, with relaxed
+// agent-scope atomic stores, to a staging vector that was filled with a signalling-NaN
+// sentinel before the launch.  Arithmetic never produces a signalling NaN (IEEE
+// operations quiet NaNs), and every published value is the result of arithmetic, so
+// a staged word is either the sentinel or its final value: the data is its own ready
+// flag.  A consumer polls the 64 values of a block with relaxed atomic loads until no
+// lane sees the sentinel; no separate flag, fence or second round trip is needed.
+template <typename E> struct Stage;
+template <> struct Stage<float> { using W = uint32_t; static constexpr int NW = 1; static constexpr W kSentinel = 0x7FA5A5A5u; };
+template <> struct Stage<double> { using W = uint64_t; static constexpr int NW = 1; static constexpr W kSentinel = 0x7FF5A5A5A5A5A5A5ull; };
+template <> struct Stage<float2> { using W = uint32_t; static constexpr int NW = 2; static constexpr W kSentinel = 0x7FA5A5A5u; };
+template <> struct Stage<double2> { using W = uint64_t; static constexpr int NW = 2; static constexpr W kSentinel = 0x7FF5A5A5A5A5A5A5ull; };
+
 template <typename E>
-__global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t incx, int unit, int *ctrl, int nb) {
+__device__ __forceinline__ bool stage_load(const typename Stage<E>::W *st, int64_t i, E &out) {
+  using S = Stage<E>;
+  typename S::W w[S::NW];
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < S::NW; ++k) {
+    w[k] = __hip_atomic_load(&st[i * S::NW + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ok = ok && w[k] != S::kSentinel;
+  }
+  __builtin_memcpy(&out, &w[0], sizeof(E));
+  return ok;
+}
+template <typename E>
+__device__ __forceinline__ void stage_store(typename Stage<E>::W *st, int64_t i, E v) {
+  using S = Stage<E>;
+  typename S::W w[S::NW];
+  __builtin_memcpy(&w[0], &v, sizeof(E));
+#pragma unroll
+  for (int k = 0; k < S::NW; ++k) __hip_atomic_store(&st[i * S::NW + k], w[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename W>
+__global__ void stage_init_kernel(W *st, int64_t nwords, W sentinel, int *ctrl) {
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i0 < 2) ctrl[i0] = 0;  // [0] ticket, [1] wait-bound error flag
+  for (int64_t i = i0; i < nwords; i += (int64_t)gridDim.x * blockDim.x) st[i] = sentinel;
+}
+
+// Persistent forward solve of the virtual lower-triangular M (see the file comment).
+template <typename E>
+__global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t incx, int unit,
+                                                   typename Stage<E>::W *stage, int *ctrl, int nb) {
   __shared__ E T[NB][NB + 1];
+  __shared__ E Td[NB][NB + 1];
   __shared__ E xs[NB];
   __shared__ E part[4][NB];
   __shared__ int bsh;
@@ -158,33 +212,78 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
   __syncthreads();
   const int b = bsh;
   if (b >= nb) return;
-  int *flags = ctrl + 1;
   const int64_t i0 = (int64_t)b * NB;
-  E acc = ezero<E>();
-  // Off-diagonal tiles of this row panel stream through two register slots
-  // (slot j&1 holds tile j, loaded two tiles ahead); the unrolled-by-two loop keeps
-  // the slot indices static.
-  E reg0[16], reg1[16];
-  auto step = [&](int j, E(&cur)[16]) {
-    if (t == 0) {
-      // Tickets make this wait finite (block j belongs to a workgroup that started
-      // earlier and waits only on smaller tickets); the spin bound is a backstop
-      // that records an error instead of hanging if that invariant were broken.
-      // The agent-scope acquire makes x_j visible to the workgroup after the barrier.
-      uint32_t spins = 0;
-      while (__hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins > (1u << 26)) {
-          atomicExch(&ctrl[nb + 1], 1);
-          break;
-        }
+  // The diagonal block depends on nothing: fetch it first and invert it while the
+  // blocks this one depends on are still being solved.  Wavefront q computes columns
+  // [16q, 16q + 16) of inv(L_bb) by forward substitution on the unit vectors (lane =
+  // row; 64 steps of: finish row k, broadcast it, update the rows below), so the
+  // solve on the critical path is one triangular matrix-vector product.
+  {
+    E dreg[16];
+    load_tile(v, i0, i0, true, dreg);
+    store_tile(v, Td, dreg);
+    __syncthreads();
+    const int64_t i = i0 + lane;
+    const E d = (unit || i >= v.n) ? eone<E>() : v.get(i, i);
+    const E rec = erecip(d);
+    E Y[16];
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) Y[cc] = (lane == 16 * q + cc) ? eone<E>() : ezero<E>();
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      if (lane == k) {
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc) Y[cc] = qdiv(Y[cc], d, rec);
+      }
+      const E lik = Td[k][lane];  // L(i, k) for lane i > k
+      E yk[16];
+#pragma unroll
+      for (int cc = 0; cc < 16; ++cc) yk[cc] = rlane(Y[cc], k);
+      if (lane > k) {  // rows above k are untouched, as in substitution (no 0 * inf)
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc) Y[cc] = emad(Y[cc], eneg(lik), yk[cc]);
       }
     }
-    __syncthreads();  // x_j published; every wavefront is done reading T and xs
+    __syncthreads();  // every wavefront is done reading Td
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) Td[16 * q + cc][lane] = Y[cc];  // Td now holds inv(L_bb)
+  }
+  E acc = ezero<E>();
+  // wavefront 0 keeps the next block's staged values in flight (xnext); a block that
+  // was already complete when they arrived costs no further load latency
+  E xnext = ezero<E>();
+  int xnext_j = -1;
+  auto load_block = [&](int j, E &val) -> bool {  // true when every lane holds a final value
+    const int64_t c = (int64_t)j * NB + lane;
+    bool ok = true;
+    val = ezero<E>();
+    if (c < v.n) ok = stage_load<E>(stage, c, val);
+    return __ballot(!ok) == 0;
+  };
+  // Off-diagonal tiles stream through two register slots (slot j&1 holds tile j,
+  // loaded two tiles ahead); the unrolled-by-two loop keeps slot indices static.
+  E reg0[16], reg1[16];
+  auto step = [&](int j, E(&cur)[16]) {
+    __syncthreads();  // every wavefront is done reading T and xs of tile j-1
     store_tile(v, T, cur);
-    if (t < NB) {
-      const int64_t c = (int64_t)j * NB + t;
-      xs[t] = c < v.n ? x[v.map(c) * incx] : ezero<E>();
+    if (q == 0) {
+      E val = xnext;
+      if (xnext_j != j) {
+        // Tickets make this wait finite (block j belongs to a workgroup that started
+        // earlier and waits only on smaller tickets); the spin bound is a backstop
+        // that records an error instead of hanging if that invariant were broken.
+        uint32_t spins = 0;
+        while (!load_block(j, val)) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > (1u << 22)) {  // seconds; a legitimate wait is microseconds
+            if (lane == 0) atomicExch(&ctrl[1], 1);
+            break;
+          }
+        }
+      }
+      xs[lane] = val;
+      xnext_j = -1;
+      if (j + 1 < b && load_block(j + 1, xnext)) xnext_j = j + 1;
     }
     __syncthreads();
     if (j + 2 < b) load_tile(v, i0, (int64_t)(j + 2) * NB, false, cur);
@@ -200,28 +299,31 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
     step(j, reg0);
     if (j + 1 < b) step(j + 1, reg1);
   }
-  __syncthreads();  // last panel tile consumed before T is reused
   part[q][lane] = acc;
-  // diagonal block
-  E reg[16];
-  load_tile(v, i0, i0, true, reg);
-  store_tile(v, T, reg);
   __syncthreads();
   if (q == 0) {
     const int64_t i = i0 + lane;
     const int nrows = (int)std::min<int64_t>(NB, v.n - i0);
-    E rhs = ezero<E>(), dval = ezero<E>();
+    (void)nrows;
+    E rhs = ezero<E>();
     if (i < v.n) {
       rhs = x[v.map(i) * incx];
       rhs = esub(rhs, eadd(eadd(part[0][lane], part[1][lane]), eadd(part[2][lane], part[3][lane])));
-      if (!unit) dval = v.get(i, i);
     }
-    E xr = solve_diag_block(rhs, T, dval, unit, nrows);
-    if (i < v.n) x[v.map(i) * incx] = xr;
-    __threadfence();
+    // x = inv(L_bb) r: lane i forms row i from the broadcast residual (four partial
+    // sums in a fixed order keep the chain short and the result deterministic)
+    xs[lane] = rhs;
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wavefront's LDS writes are done
+    __builtin_amdgcn_wave_barrier();
+    E s4[4] = {ezero<E>(), ezero<E>(), ezero<E>(), ezero<E>()};
+#pragma unroll
+    for (int c = 0; c < NB; ++c) s4[c & 3] = emad(s4[c & 3], Td[c][lane], xs[c]);
+    const E xr = eadd(eadd(s4[0], s4[1]), eadd(s4[2], s4[3]));
+    if (i < v.n) {
+      stage_store<E>(stage, i, xr);  // publish first: it is the critical path
+      x[v.map(i) * incx] = xr;
+    }
   }
-  __syncthreads();
-  if (t == 0) __hip_atomic_store(&flags[b], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void zero_ints_kernel(int *p, int count) {
@@ -318,15 +420,18 @@ static int trsv_impl(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const
   CORAL_TRY(ensure_init());
   hipStream_t s = resolve(stream);
   const int nb = (int)cdiv(n, NB);
-  void *ctrl = nullptr;
-  // ctrl: [0] ticket, [1..nb] block flags, [nb+1] wait-bound error flag
-  CORAL_TRY(workspace(s, WS_SMALL, sizeof(int) * (size_t)(nb + 2), &ctrl));
-  hipLaunchKernelGGL(zero_ints_kernel, dim3((unsigned)std::min<int64_t>(cdiv(nb + 2, 256), 64)), dim3(256), 0, s,
-                     (int *)ctrl, nb + 2);
+  using W = typename Stage<E>::W;
+  void *ctrl = nullptr, *stage = nullptr;
+  // ctrl: [0] ticket, [1] wait-bound error flag; stage: published x, sentinel-filled
+  CORAL_TRY(workspace(s, WS_SMALL, 2 * sizeof(int), &ctrl));
+  const int64_t nwords = n * Stage<E>::NW;
+  CORAL_TRY(workspace(s, WS_AUX2, sizeof(W) * (size_t)nwords, &stage));
+  hipLaunchKernelGGL(stage_init_kernel<W>, dim3((unsigned)std::min<int64_t>(cdiv(nwords, 256), 1024)), dim3(256), 0, s,
+                     (W *)stage, nwords, Stage<E>::kSentinel, (int *)ctrl);
   CORAL_LAUNCH_CHECK();
   TriView<E> v = make_view(uplo, trans, a, lda, n);
   hipLaunchKernelGGL(trsv_kernel<E>, dim3((unsigned)nb), dim3(256), 0, s, v, x, incx,
-                     (int)(diag == CORAL_UNIT_DIAGONAL), (int *)ctrl, nb);
+                     (int)(diag == CORAL_UNIT_DIAGONAL), (W *)stage, (int *)ctrl, nb);
   CORAL_LAUNCH_CHECK();
   return CORAL_OK;
 }
@@ -408,7 +513,10 @@ __global__ __launch_bounds__(256) void trsv_batched_kernel(int uplo, int trans, 
           rhs = xsol[i] - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
           if (!unit) dval = v.get(i, i);
         }
-        double xr = solve_diag_block(rhs, T, dval, unit, nrows);
+        double col[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) col[c] = T[c][lane];
+        double xr = solve_diag_block(rhs, [&](int c) { return col[c]; }, dval, unit, nrows);
         if (i < n) xsol[i] = xr;
       }
       __syncthreads();
