@@ -9,8 +9,8 @@
 //     vector); each thread keeps NV*EV row sums and reads NV vectors of every
 //     column it visits, four columns in flight.  Column chunks write partial row
 //     sums, which a second kernel adds in chunk order and combines with alpha/beta.
-//  T/C: one wavefront per column; lanes walk the column two vectors at a time and
-//     the dot product is finished with a DPP/shuffle butterfly.  Long columns are
+//  T/C: a wavefront owns four columns and reads each x vector once for all four;
+//     the dot products are finished with a DPP/shuffle butterfly.  Long columns are
 //     split into row chunks whose partial dots are reduced the same way.
 //  A is streamed once, with nontemporal loads.
 #include "coral_common.hpp"
@@ -102,40 +102,57 @@ __global__ __launch_bounds__(256) void gemv_n_kernel(int64_t m, int64_t n, const
       if (e < avail[v]) out[row[v] + e] = acc[v][e];
 }
 
-// T/C: one wavefront per column; per step lane l reads vectors at
-// i + v*64*EV + l*EV (v < kTVecs) of the column and of x.
-constexpr int kTVecs = 4;
+// T/C: a wavefront owns kTCols consecutive columns; per step lane l reads the
+// vectors at i + v*64*EV + l*EV (v < kTVecs) of x once and of each of its columns,
+// so x is read once per kTCols columns.
+constexpr int kTVecs = 2, kTCols = 4;
 template <typename E, bool CONJ>
 __global__ __launch_bounds__(256) void gemv_t_kernel(int64_t m, int64_t n, const E *__restrict__ A, int64_t lda,
                                                      const E *__restrict__ x, int64_t incx, E *__restrict__ P,
                                                      int64_t rpc, int vec_ok, int xvec_ok) {
-  constexpr int EV = 16 / sizeof(E), NV = kTVecs;
+  constexpr int EV = 16 / sizeof(E), NV = kTVecs, CPW = kTCols;
   const int lane = threadIdx.x & 63;
-  const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (j >= n) return;  // uniform per wavefront; no block-level barrier below
+  const int64_t j0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * CPW;
+  if (j0 >= n) return;  // uniform per wavefront; no block-level barrier below
   const int64_t r0 = (int64_t)blockIdx.y * rpc;
   const int64_t r1 = r0 + rpc < m ? r0 + rpc : m;
-  const E *col = A + j * lda;
-  E acc = ezero<E>();
+  const int ncols = n - j0 < CPW ? (int)(n - j0) : CPW;
+  E acc[CPW];
+#pragma unroll
+  for (int u = 0; u < CPW; ++u) acc[u] = ezero<E>();
   for (int64_t i = r0; i < r1; i += 64 * EV * NV) {
+    E xv[NV][EV], a[NV][CPW][EV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const int64_t rr = i + v * 64 * EV + lane * EV;
       const int64_t avail = r1 - rr;
-      E a[EV], xv[EV];
-      load_vec<E>(a, col + rr, vec_ok, avail);
       if (incx == 1) {
-        load_vec<E>(xv, x + rr, xvec_ok, avail);
+        load_vec<E>(xv[v], x + rr, xvec_ok, avail);
       } else {
 #pragma unroll
-        for (int e = 0; e < EV; ++e) xv[e] = (e < avail) ? x[(rr + e) * incx] : ezero<E>();
+        for (int e = 0; e < EV; ++e) xv[v][e] = (e < avail) ? x[(rr + e) * incx] : ezero<E>();
       }
 #pragma unroll
-      for (int e = 0; e < EV; ++e) acc = emad<CONJ>(acc, a[e], xv[e]);
+      for (int u = 0; u < CPW; ++u) {
+        if (u < ncols) load_vec<E>(a[v][u], A + (j0 + u) * lda + rr, vec_ok, avail);
+        else {
+#pragma unroll
+          for (int e = 0; e < EV; ++e) a[v][u][e] = ezero<E>();
+        }
+      }
     }
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int u = 0; u < CPW; ++u)
+#pragma unroll
+        for (int e = 0; e < EV; ++e) acc[u] = emad<CONJ>(acc[u], a[v][u][e], xv[v][e]);
   }
-  acc = wave_esum(acc);
-  if (lane == 0) P[(int64_t)blockIdx.y * n + j] = acc;
+#pragma unroll
+  for (int u = 0; u < CPW; ++u) {
+    const E sum = wave_esum(acc[u]);
+    if (lane == 0 && u < ncols) P[(int64_t)blockIdx.y * n + j0 + u] = sum;
+  }
 }
 
 // y[i] := alpha * sum_c P[c][i] + beta * y[i].  A workgroup takes 64 rows; its four
@@ -256,7 +273,7 @@ int gemv_impl(int32_t trans, int64_t m, int64_t n, E alpha, const E *A, int64_t 
                        (const E *)P, alpha, beta, (int)is_zero_h(beta), y, incy);
     CORAL_LAUNCH_CHECK();
   } else {
-    const int64_t col_blocks = cdiv(n, 4);
+    const int64_t col_blocks = cdiv(n, 4 * kTCols);
     const int64_t unit = 64 * kTVecs * (16 / (int64_t)sizeof(E));  // rows per wave step
     int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(cdiv(8 * cus, col_blocks), cdiv(m, 4 * unit)));
     const int64_t rpc = cdiv(cdiv(m, chunks), unit) * unit;
