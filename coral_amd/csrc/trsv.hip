@@ -513,10 +513,7 @@ __global__ __launch_bounds__(256) void trsv_batched_kernel(int uplo, int trans, 
           rhs = xsol[i] - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
           if (!unit) dval = v.get(i, i);
         }
-        double col[NB];
-#pragma unroll
-        for (int c = 0; c < NB; ++c) col[c] = T[c][lane];
-        double xr = solve_diag_block(rhs, [&](int c) { return col[c]; }, dval, unit, nrows);
+        double xr = solve_diag_block(rhs, [&](int c) { return T[c][lane]; }, dval, unit, nrows);
         if (i < n) xsol[i] = xr;
       }
       __syncthreads();
