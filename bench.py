@@ -105,28 +105,52 @@ def profiled_traffic():
     return None
 
 
-def cpu_baseline(n=3072, runs=3):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_leg(fn, work, unit, scale, sample, runs=5):
+    """Times fn (a call into the oracle's C restatement of CORAL's CPU algorithm) on one
+    pinned host core: one warm-up run, then the median of `runs` runs.  `work` is the
+    algorithmic flops or bytes of one call and `scale` converts work/s into `unit`."""
+    try:
+        prev = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {sorted(prev)[0]})
+    except (AttributeError, OSError):
+        prev = None
+    try:
+        fn()
+        times = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t0)
+    finally:
+        if prev is not None:
+            os.sched_setaffinity(0, prev)
+    t = statistics.median(times)
+    return {"value": round(work / t / scale, 4), "unit": unit, "cores": 1, "kind": "port",
+            "sample": f"{sample}; oracle/coral_port.c (CORAL's algorithm restated in C), 1 pinned core of "
+                      f"{os.cpu_count()} ({_cpu_model()}), median of {runs} runs after 1 warm-up, {t:.3f} s per run"}
+
+
+def cpu_baseline(n=3072, runs=5):
     """CORAL's CPU SGEMM algorithm (oracle restatement) on one core."""
     from oracle import port
 
-    try:
-        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
-    except (AttributeError, OSError):
-        pass
     rng = np.random.default_rng(0)
     a = rng.standard_normal(n * n).astype(np.float32)
     b = rng.standard_normal(n * n).astype(np.float32)
     c = np.zeros(n * n, np.float32)
-    times = []
-    for _ in range(runs):
-        t0 = time.perf_counter()
-        port.sgemm(0, 0, n, n, n, 1.0, a, n, b, n, 0.0, c, n)
-        times.append(time.perf_counter() - t0)
-    t = statistics.median(times)
-    return {"value": 2.0 * n ** 3 / t / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
-            "sample": f"CORAL SGEMM NN algorithm restated in C (oracle/coral_port.c: MC/NC/KC 384/576/256, "
-                      f"8x12 FMA microkernel), m=n=k={n} f32 N(0,1), median of {runs} runs on 1 host core "
-                      f"({os.cpu_count()} visible), {t:.2f} s per run"}
+    return cpu_leg(lambda: port.sgemm(0, 0, n, n, n, 1.0, a, n, b, n, 0.0, c, n), 2.0 * n ** 3, "TFLOP/s", 1e12,
+                   f"SGEMM NN m=n=k={n} f32 N(0,1) (MC/NC/KC 384/576/256, 8x12 FMA microkernel)", runs)
 
 
 # ---------------------------------------------------------------------------
@@ -168,20 +192,21 @@ def suite_c1():
     call("coral_sgemm", 0, 0, n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.5, C.ptr, n, None)
     out = C.to_numpy()
     po = c.copy()
-    t0 = time.perf_counter()
-    port.sgemm(0, 0, n, n, n, 1.0, a, n, b, n, 0.5, po, n)  # cpu_baseline leg: CORAL's algorithm restated
-    cpu_s = time.perf_counter() - t0
+    port.sgemm(0, 0, n, n, n, 1.0, a, n, b, n, 0.5, po, n)
     ref = (a.reshape(n, n).T.astype(np.float64) @ b.reshape(n, n).T.astype(np.float64)) + 0.5 * c.reshape(n, n).T
     err = np.linalg.norm(out.reshape(n, n).T - ref) / (n * 2.0 ** -24 * np.linalg.norm(a) * np.linalg.norm(b))
+    pc = c.copy()
+    cpu = cpu_leg(lambda: port.sgemm(0, 0, n, n, n, 1.0, a, n, b, n, 0.5, pc, n), 2.0 * n ** 3, "GFLOP/s", 1e9,
+                  "SGEMM NN 512^3, the full configuration", runs=9)
     _emit({"config": "C1 SGEMM NN 512^3 alpha=1 beta=0.5 U[-1,1]", "ms": round(ms, 4),
            "tflops": round(2 * n ** 3 / ms / 1e9, 2), "normwise_bound_ratio": float(err),
-           "max_abs_diff_vs_oracle": float(np.abs(out - po).max()),
-           "cpu_baseline_port_gflops_1core": round(2 * n ** 3 / cpu_s / 1e9, 2)})
+           "max_abs_diff_vs_oracle": float(np.abs(out - po).max()), "cpu_baseline": cpu})
 
 
 def suite_c2():
     from coral_amd import device, synthetic
     from coral_amd._lib import call
+    from oracle import port
 
     n = 8192
     A, B, C = (device.DeviceArray(n * n, np.float64) for _ in range(3))
@@ -199,9 +224,14 @@ def suite_c2():
         rows = np.random.default_rng(2).choice(n, 64, replace=False)
         err = _fro_sample_err(Ah, Bh, Ch, rows, n, eps=2.0 ** -53, ops=ops)
         tf = 2 * n ** 3 / ms / 1e9
+        nc = 1536  # bounded CPU sample
+        ca, cb = (np.random.default_rng(20 + i).standard_normal(nc * nc) for i in range(2))
+        cc = np.zeros(nc * nc)
+        cpu = cpu_leg(lambda: port.dgemm(ops[0], ops[1], nc, nc, nc, 1.0, ca, nc, cb, nc, 0.0, cc, nc),
+                      2.0 * nc ** 3, "GFLOP/s", 1e9, f"DGEMM {'NN' if ops == (0, 0) else 'TT'} {nc}^3 f64 N(0,1)")
         _emit({"config": f"C2 DGEMM {'NN' if ops == (0, 0) else 'TT'} 8192^3 N(0,1)", "ms": round(ms, 3),
                "tflops": round(tf, 2), "frac_fp64_peak": round(tf / 78.6, 4),
-               "kernel_tflops": round(2 * n ** 3 / k_ms / 1e9, 2), "error_vs_fp64": err})
+               "kernel_tflops": round(2 * n ** 3 / k_ms / 1e9, 2), "error_vs_fp64": err, "cpu_baseline": cpu})
         del Ah, Bh, Ch
     del A, B, C
     # DGEMV N 16384^2
@@ -219,10 +249,16 @@ def suite_c2():
         k_ms, _ = kernel_avg_ms(1)
         call("coral_timing_enable", 0)
         by = 8.0 * (m * m + 2 * m)
+        mc = 8192  # bounded CPU sample (512 MiB matrix)
+        if tr == 0:
+            crng = np.random.default_rng(21)
+            cA, cx, cy = crng.uniform(-1, 1, mc * mc), crng.uniform(-1, 1, mc), np.zeros(mc)
+        cpu = cpu_leg(lambda: port.dgemv(tr, mc, mc, 1.0, cA, mc, cx, 1, 0.0, cy, 1), 8.0 * (mc * mc + 2 * mc),
+                      "GB/s", 1e9, f"DGEMV {'N' if tr == 0 else 'T'} {mc}^2 f64 U[-1,1]")
         _emit({"config": f"C2 DGEMV {'N' if tr == 0 else 'T'} 16384^2 U[-1,1] beta=0", "us": round(ms * 1e3, 1),
                "gbps": round(by / ms / 1e6, 1), "kernel_gbps": round(8.0 * (m * m + m) / k_ms / 1e6, 1),
-               "frac_hbm": round(by / ms / 1e6 / PEAK_HBM, 4)})
-    del GA, gx, gy
+               "frac_hbm": round(by / ms / 1e6 / PEAK_HBM, 4), "cpu_baseline": cpu})
+    del GA, gx, gy, cA
     # DTRSV lower non-unit 8192
     n = 8192
     rng = np.random.default_rng(3)
@@ -245,16 +281,26 @@ def suite_c2():
     x = X.to_numpy()
     berr = np.linalg.norm(b - Ah @ x) / (np.linalg.norm(Ah) * np.linalg.norm(x) + np.linalg.norm(b))
     by = 8.0 * (n * (n + 1) / 2 + 2 * n)
+    af = np.asfortranarray(Ah).reshape(-1, order="F")
+    xc = b.copy()
+    cpu = cpu_leg(lambda: (xc.__setitem__(slice(None), b), port.dtrsv(1, 0, 1, n, af, n, xc, 1)), by, "GB/s", 1e9,
+                  "DTRSV lower non-unit n=8192, the full configuration (NB=8 substitution + fused updates)")
     _emit({"config": "C2 DTRSV lower non-unit 8192", "us": round(ms * 1e3, 1), "gbps": round(by / ms / 1e6, 1),
-           "gflops": round(n * n / ms / 1e6, 2), "backward_error": float(berr)})
+           "gflops": round(n * n / ms / 1e6, 2), "backward_error": float(berr), "cpu_baseline": cpu})
 
 
 def suite_c3():
     from coral_amd import device, synthetic
     from coral_amd._lib import call
+    from oracle import port
 
+    nc = 2048  # bounded CPU sample, shared by the C3 lines
+    ca, cb = (np.random.default_rng(30 + i).standard_normal(nc * nc).astype(np.float32) for i in range(2))
+    cc = np.zeros(nc * nc, np.float32)
+    cpu = cpu_leg(lambda: port.sgemm(0, 0, nc, nc, nc, 1.0, ca, nc, cb, nc, 0.0, cc, nc), 2.0 * nc ** 3, "GFLOP/s",
+                  1e9, f"SGEMM NN {nc}^3 f32 N(0,1)")
     for n in (4096, 8192, 16384):
-        A, B, C, C2 = (device.DeviceArray(n * n, np.float32) for _ in range(4))
+        A, B, C, C2, C3 = (device.DeviceArray(n * n, np.float32) for _ in range(5))
         for dist in ("normal", "wide"):
             device.fill_random(A, synthetic.seed_for(3, "A") + n, "normal")
             device.fill_random(B, synthetic.seed_for(3, "B") + n, "normal")
@@ -269,15 +315,19 @@ def suite_c3():
                                            n, None), steps, 1)
             msp = timed_steps(lambda: call("coral_sgemm", 0, 0, n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.0, C2.ptr, n,
                                            None), steps, 1)
-            Ah, Bh, Cb, Cp = (_host_mat(x, n, n, np.float32) for x in (A, B, C, C2))
+            msf = timed_steps(lambda: call("coral_sgemm_fp16x3", 0, 0, n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.0, C3.ptr,
+                                           n, None), steps, 1)
+            Ah, Bh, Cb, Cp, Cf = (_host_mat(x, n, n, np.float32) for x in (A, B, C, C2, C3))
             rows = np.random.default_rng(5).choice(n, 128, replace=False)
             eb3 = _fro_sample_err(Ah, Bh, Cb, rows, n)
             ep = _fro_sample_err(Ah, Bh, Cp, rows, n)
+            ef = _fro_sample_err(Ah, Bh, Cf, rows, n)
             _emit({"config": f"C3 SGEMM NN {n}^3 {dist}", "bf16x3_tflops": round(2 * n ** 3 / ms3 / 1e9, 2),
                    "plain_fp32_tflops": round(2 * n ** 3 / msp / 1e9, 2), "ratio": round(msp / ms3, 3),
-                   "error_bf16x3": eb3, "error_plain": ep})
-            del Ah, Bh, Cb, Cp
-        del A, B, C, C2
+                   "fp16x3_tflops": round(2 * n ** 3 / msf / 1e9, 2), "fp16x3_ratio": round(msp / msf, 3),
+                   "error_bf16x3": eb3, "error_plain": ep, "error_fp16x3": ef, "cpu_baseline": cpu})
+            del Ah, Bh, Cb, Cp, Cf
+        del A, B, C, C2, C3
 
 
 def _log_uniform(rng, count, lo, hi):
@@ -289,6 +339,7 @@ def suite_c4():
 
     from coral_amd import device
     from coral_amd._lib import call
+    from oracle import port
 
     rng = np.random.default_rng(4)
     for dt, name, single in ((np.float32, "coral_sgemm_batched", "coral_sgemm"),
@@ -343,9 +394,19 @@ def suite_c4():
         loop_out = C.to_numpy()
         diff = float(np.abs(outs - loop_out).max() / max(np.abs(loop_out).max(), 1e-30))
         flops = float((2 * dims[:, 0] * dims[:, 1] * dims[:, 2]).sum())
+        # CPU leg: CORAL's algorithm looped over the first 300 problems of the same batch
+        sub = dims[:300]
+        cpu_ops = [(int(m), int(n), int(k), np.random.default_rng(int(i)).standard_normal(int(m * k)).astype(dt),
+                    np.random.default_rng(int(i) + 1).standard_normal(int(k * n)).astype(dt),
+                    np.zeros(int(m * n), dt)) for i, (m, n, k) in enumerate(sub)]
+        pfn = port.sgemm if dt == np.float32 else port.dgemm
+        cpu = cpu_leg(lambda: [pfn(0, 0, m, n, k, 1.0, a, m, b, k, 0.0, c, m) for (m, n, k, a, b, c) in cpu_ops],
+                      float((2 * sub[:, 0] * sub[:, 1] * sub[:, 2]).sum()), "GFLOP/s", 1e9,
+                      "the first 300 problems of the batch, one CORAL call each")
         _emit({"config": f"C4 batched {'S' if dt == np.float32 else 'D'}GEMM 10000 problems log-U[8,512] NN",
                "graph_ms": round(ms_graph, 3), "loop_ms": round(ms_loop, 3), "speedup": round(ms_loop / ms_graph, 2),
-               "tflops": round(flops / ms_graph / 1e9, 2), "flops": flops, "max_rel_diff_vs_loop": diff})
+               "tflops": round(flops / ms_graph / 1e9, 2), "flops": flops, "max_rel_diff_vs_loop": diff,
+               "cpu_baseline": cpu})
         call("coral_graph_destroy", g.value)
         stream.destroy()
     # batched DTRSV: 4096 lower non-unit, n log-U[32,256]
@@ -382,14 +443,20 @@ def suite_c4():
     e1.record()
     e1.synchronize()
     ms_l = e0.elapsed_ms(e1)
+    cx = [x.copy() for x in xs[:500]]
+    cpu = cpu_leg(lambda: [(c.__setitem__(slice(None), x), port.dtrsv(1, 0, 1, int(n), a, int(n), c, 1))
+                           for n, a, x, c in zip(ns[:500], mats[:500], xs[:500], cx)],
+                  float((ns[:500].astype(float) ** 2).sum()), "GFLOP/s", 1e9,
+                  "the first 500 problems of the batch, one CORAL call each")
     _emit({"config": "C4 batched DTRSV 4096 lower non-unit n log-U[32,256]", "batched_ms": round(ms_b, 3),
            "loop_ms": round(ms_l, 3), "speedup": round(ms_l / ms_b, 2),
-           "gflops": round(float((ns.astype(float) ** 2).sum()) / ms_b / 1e6, 2)})
+           "gflops": round(float((ns.astype(float) ** 2).sum()) / ms_b / 1e6, 2), "cpu_baseline": cpu})
 
 
 def suite_c5():
     from coral_amd import device, synthetic
     from coral_amd._lib import call
+    from oracle import port
 
     n = 4096
     A, B, C = (device.DeviceArray(n * n, np.complex64) for _ in range(3))
@@ -401,11 +468,20 @@ def suite_c5():
     zero = (ctypes.c_float * 2)(0.0, 0.0)
     Ah, Bh = _host_mat(A, n, n, np.complex64), _host_mat(B, n, n, np.complex64)
     rows = np.random.default_rng(6).choice(n, 64, replace=False)
-    for name in ("coral_cgemm", "coral_cgemm_3m_bf16x3"):
+    nc = 1024  # bounded CPU sample
+    crng = np.random.default_rng(50)
+    ca, cb = ((crng.uniform(-1, 1, nc * nc) + 1j * crng.uniform(-1, 1, nc * nc)).astype(np.complex64)
+              for _ in range(2))
+    cc = np.zeros(nc * nc, np.complex64)
+    cpu = cpu_leg(lambda: port.cgemm(2, 2, nc, nc, nc, (1.0, 0.0), ca, nc, cb, nc, (0.0, 0.0), cc, nc),
+                  8.0 * nc ** 3, "GFLOP/s", 1e9, f"CGEMM CC {nc}^3 complex64 U[-1,1] (4M FMA microkernel)")
+    for name in ("coral_cgemm", "coral_cgemm_3m_bf16x3", "coral_cgemm_3m_fp16x3"):
         ms = timed_steps(lambda: call(name, 2, 2, n, n, n, one, A.ptr, n, B.ptr, n, zero, C.ptr, n, None), 5, 1)
         err = _fro_sample_err(Ah, Bh, _host_mat(C, n, n, np.complex64), rows, n, cplx=True, ops=(2, 2))
-        _emit({"config": f"C5 {'CGEMM 4M FP32-MFMA' if name == 'coral_cgemm' else 'CGEMM 3M BF16x3'} CC 4096^3",
-               "ms": round(ms, 3), "tflops": round(8 * n ** 3 / ms / 1e9, 2), "error_vs_fp64": err})
+        label = {"coral_cgemm": "CGEMM 4M FP32-MFMA", "coral_cgemm_3m_bf16x3": "CGEMM 3M BF16x3",
+                 "coral_cgemm_3m_fp16x3": "CGEMM 3M FP16x3"}[name]
+        _emit({"config": f"C5 {label} CC 4096^3", "ms": round(ms, 3), "tflops": round(8 * n ** 3 / ms / 1e9, 2),
+               "error_vs_fp64": err, "cpu_baseline": cpu})
     del A, B, C, Ah, Bh
     A, B, C = (device.DeviceArray(n * n, np.complex128) for _ in range(3))
     device.fill_random(A, synthetic.seed_for(5, "A"), "uniform")
@@ -414,8 +490,13 @@ def suite_c5():
     zerod = (ctypes.c_double * 2)(0.0, 0.0)
     ms = timed_steps(lambda: call("coral_zgemm", 0, 0, n, n, n, oned, A.ptr, n, B.ptr, n, zerod, C.ptr, n, None), 3, 1)
     tf = 8 * n ** 3 / ms / 1e9
+    nz = 768
+    za, zb = ((crng.uniform(-1, 1, nz * nz) + 1j * crng.uniform(-1, 1, nz * nz)) for _ in range(2))
+    zc = np.zeros(nz * nz, np.complex128)
+    cpu = cpu_leg(lambda: port.zgemm(0, 0, nz, nz, nz, (1.0, 0.0), za, nz, zb, nz, (0.0, 0.0), zc, nz),
+                  8.0 * nz ** 3, "GFLOP/s", 1e9, f"ZGEMM NN {nz}^3 complex128 U[-1,1]")
     _emit({"config": "C5 ZGEMM NN 4096^3 FP64-MFMA", "ms": round(ms, 3), "tflops": round(tf, 2),
-           "frac_fp64_peak": round(tf / 78.6, 4)})
+           "frac_fp64_peak": round(tf / 78.6, 4), "cpu_baseline": cpu})
     del A, B, C
     m = 16384
     Mh = device.DeviceArray(m * m, np.complex64)
@@ -425,8 +506,15 @@ def suite_c5():
     device.fill_random(x, synthetic.seed_for(5, "x"), "uniform")
     ms = timed_steps(lambda: call("coral_chemv", 0, m, one, Mh.ptr, m, x.ptr, 1, zero, y.ptr, 1, None), 10, 2)
     by = 8.0 * m * (m + 1) / 2 + 3 * 8.0 * m
+    mh = 8192
+    ha = (crng.uniform(-1, 1, mh * mh) + 1j * crng.uniform(-1, 1, mh * mh)).astype(np.complex64)
+    hx = (crng.uniform(-1, 1, mh) + 1j * crng.uniform(-1, 1, mh)).astype(np.complex64)
+    hy = np.zeros(mh, np.complex64)
+    cpu = cpu_leg(lambda: port.chemv(0, mh, (1.0, 0.0), ha, mh, hx, 1, (0.0, 0.0), hy, 1),
+                  8.0 * mh * (mh + 1) / 2 + 3 * 8.0 * mh, "GB/s", 1e9,
+                  f"CHEMV upper n={mh} complex64 (the scalar lda==n loop)")
     _emit({"config": "C5 CHEMV upper 16384 (stored triangle of a random complex64 matrix)", "us": round(ms * 1e3, 1),
-           "gbps": round(by / ms / 1e6, 1), "frac_hbm": round(by / ms / 1e6 / PEAK_HBM, 4)})
+           "gbps": round(by / ms / 1e6, 1), "frac_hbm": round(by / ms / 1e6 / PEAK_HBM, 4), "cpu_baseline": cpu})
 
 
 def run_suite(which):
