@@ -54,15 +54,6 @@ struct TF32 {  // FP32 MFMA 16x16x4, plain SGEMM / CGEMM 4M
   }
 };
 
-// IEEE binary32 -> bfloat16, round to nearest even (NaN stays a quiet NaN).
-__device__ __forceinline__ uint32_t bf16_rn_bits(float v) {
-  uint32_t u = __float_as_uint(v);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return u >> 16;
-}
-__device__ __forceinline__ float bf16_bits_to_f32(uint32_t h) { return __uint_as_float(h << 16); }
-
 // Split forms: x = hi + lo + r with hi = rn(x), lo = rn(x - hi) in a 16-bit type;
 // x - hi is exact in fp32.  The operand is first scaled by a power of two per row
 // of op(A) / column of op(B) so that its largest magnitude has exponent SCALE_EXP.
@@ -74,8 +65,10 @@ struct TBF3 {  // BF16 hi/lo planes (8+8 significand bits), three 16x16x32 BF16 
   static constexpr bool ROWS_INTERLEAVED = false;
   static constexpr int SCALE_EXP = 0;  // BF16 has FP32's exponent range: scale only to centre it
   __device__ static __forceinline__ void split(float x, uint32_t &hi, uint32_t &lo) {
-    hi = bf16_rn_bits(x);
-    lo = bf16_rn_bits(x - bf16_bits_to_f32(hi));
+    const __bf16 h = (__bf16)x;  // v_cvt_pk_bf16_f32: round to nearest even
+    const __bf16 l = (__bf16)(x - (float)h);
+    hi = __builtin_bit_cast(uint16_t, h);
+    lo = __builtin_bit_cast(uint16_t, l);
   }
   __device__ static __forceinline__ void mma(acc_t &c, const v4u *a, const v4u *b) {
     bf16x8 ah = __builtin_bit_cast(bf16x8, a[0]);
