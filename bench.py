@@ -517,13 +517,150 @@ def suite_c5():
            "gbps": round(by / ms / 1e6, 1), "frac_hbm": round(by / ms / 1e6 / PEAK_HBM, 4), "cpu_baseline": cpu})
 
 
+def suite_rows():
+    """The remaining routines of the call surface, one line each, at sizes that
+    stream well past the caches: GB/s of algorithmic bytes against HBM, with the
+    CORAL algorithm on one core beside each on a bounded sample."""
+    import ctypes
+
+    from coral_amd import device
+    from coral_amd._lib import call
+    from oracle import port
+
+    cplx_arg = {np.complex64: ctypes.c_float * 2, np.complex128: ctypes.c_double * 2}
+
+    def scal(dt, v):
+        if dt in cplx_arg:
+            return cplx_arg[dt](float(np.real(v)), float(np.imag(v)))
+        return float(v)
+
+    def emit(label, ms, nbytes, cpu=None, extra=None):
+        d = {"config": f"rows {label}", "us": round(ms * 1e3, 1), "gbps": round(nbytes / ms / 1e6, 1),
+             "frac_hbm": round(nbytes / ms / 1e6 / PEAK_HBM, 4)}
+        if extra:
+            d.update(extra)
+        if cpu:
+            d["cpu_baseline"] = cpu
+        _emit(d)
+
+    def host_rand(dt, count, seed):
+        r = np.random.default_rng(seed)
+        v = r.uniform(-1, 1, count)
+        if dt in (np.complex64, np.complex128):
+            v = v + 1j * r.uniform(-1, 1, count)
+        return v.astype(dt)
+
+    # ---- GEMV (a8, a9): N and T/C ----
+    for dt, p, m in ((np.float32, "s", 16384), (np.complex64, "c", 8192), (np.complex128, "z", 8192)):
+        es = np.dtype(dt).itemsize
+        A, x, y = device.DeviceArray(m * m, dt), device.DeviceArray(m, dt), device.DeviceArray(m, dt)
+        device.fill_random(A, 71, "uniform")
+        device.fill_random(x, 72, "uniform")
+        hs = 2048 if p == "s" else 1024
+        ha, hx, hy = host_rand(dt, hs * hs, 73), host_rand(dt, hs, 74), np.zeros(hs, dt)
+        for tr in (0, 2 if p != "s" else 1):
+            ms = timed_steps(lambda: call(f"coral_{p}gemv", tr, m, m, scal(dt, 1.0), A.ptr, m, x.ptr, 1, scal(dt, 0.0),
+                                          y.ptr, 1, None), 20, 3)
+            cpu = cpu_leg(lambda: getattr(port, f"{p}gemv")(tr, hs, hs, 1.0, ha, hs, hx, 1, 0.0, hy, 1),
+                          es * (hs * hs + 2 * hs), "GB/s", 1e9, f"{p.upper()}GEMV {'NTC'[tr]} {hs}^2")
+            emit(f"{p.upper()}GEMV {'NTC'[tr]} {m}^2", ms, es * (m * m + 2 * m), cpu)
+        del A, x, y
+    # ---- SYMV / HEMV (a10, a11): stored triangle ----
+    for dt, name, n, uplo in ((np.float32, "ssymv", 16384, 0), (np.float64, "dsymv", 16384, 0),
+                              (np.complex128, "zhemv", 8192, 1)):
+        es = np.dtype(dt).itemsize
+        A, x, y = device.DeviceArray(n * n, dt), device.DeviceArray(n, dt), device.DeviceArray(n, dt)
+        device.fill_random(A, 75, "uniform")
+        device.fill_random(x, 76, "uniform")
+        ms = timed_steps(lambda: call(f"coral_{name}", uplo, n, scal(dt, 1.0), A.ptr, n, x.ptr, 1, scal(dt, 0.0),
+                                      y.ptr, 1, None), 20, 3)
+        hs = 2048
+        ha, hx, hy = host_rand(dt, hs * hs, 77), host_rand(dt, hs, 78), np.zeros(hs, dt)
+        cpu = cpu_leg(lambda: getattr(port, name)(uplo, hs, 1.0, ha, hs, hx, 1, 0.0, hy, 1),
+                      es * (hs * (hs + 1) / 2 + 3 * hs), "GB/s", 1e9, f"{name.upper()} n={hs}")
+        emit(f"{name.upper()} {'upper' if uplo == 0 else 'lower'} {n}", ms, es * (n * (n + 1) / 2 + 3 * n), cpu)
+        del A, x, y
+    # ---- TRSV / TRMV (a12-a14): strictly-lower U[-1,1]/n, diagonal U[1,2] ----
+    n = 8192
+    rng = np.random.default_rng(79)
+    base = np.tril(rng.uniform(-1, 1, (n, n)) / n, -1) + np.diag(rng.uniform(1, 2, n))
+    for dt, p in ((np.float32, "s"), (np.float64, "d"), (np.complex64, "c"), (np.complex128, "z")):
+        es = np.dtype(dt).itemsize
+        Ah = base.astype(dt)
+        if p in "cz":
+            Ah = Ah + 1j * np.tril(rng.uniform(-1, 1, (n, n)) / n, -1).astype(dt)
+        af = np.asfortranarray(Ah).reshape(-1, order="F")
+        A = device.DeviceArray.from_numpy(af)
+        b = host_rand(dt, n, 80)
+        X = device.DeviceArray.from_numpy(b)
+        nbytes = es * (n * (n + 1) / 2 + 2 * n)
+        for trans in (0, 1):
+            for routine in ("trsv", "trmv"):
+                ts = []
+                e0, e1 = device.Event(), device.Event()
+                for _ in range(7):
+                    X.upload(b)
+                    device.synchronize()
+                    e0.record()
+                    call(f"coral_{p}{routine}", 1, trans, 1, n, A.ptr, n, X.ptr, 1, None)
+                    e1.record()
+                    e1.synchronize()
+                    ts.append(e0.elapsed_ms(e1))
+                ms = statistics.median(ts[2:])
+                extra = None
+                if routine == "trsv":
+                    xs = X.to_numpy().astype(np.complex128 if p in "cz" else np.float64)
+                    M = Ah if trans == 0 else Ah.T
+                    bb = b.astype(xs.dtype)
+                    extra = {"backward_error": float(np.linalg.norm(bb - M @ xs) /
+                                                     (np.linalg.norm(M) * np.linalg.norm(xs) + np.linalg.norm(bb)))}
+                cpu = None
+                if p == "d":
+                    xc = b.copy()
+                    cpu = cpu_leg(lambda: (xc.__setitem__(slice(None), b),
+                                           getattr(port, f"d{routine}")(1, trans, 1, n, af, n, xc, 1)),
+                                  nbytes, "GB/s", 1e9, f"D{routine.upper()} lower {'NT'[trans]} n={n}")
+                emit(f"{p.upper()}{routine.upper()} lower {'NT'[trans]} non-unit {n}", ms, nbytes, cpu, extra)
+        del A, X, Ah, af
+    # ---- level 1 (a15-a19), n = 2^27 ----
+    n = 1 << 27
+    hn = 1 << 22
+    for dt, p in ((np.float32, "s"), (np.float64, "d"), (np.complex128, "z")):
+        es = np.dtype(dt).itemsize
+        x, y = device.DeviceArray(n, dt), device.DeviceArray(n, dt)
+        device.fill_random(x, 81, "uniform")
+        device.fill_random(y, 82, "uniform")
+        res = device.DeviceArray(2, np.float64)
+        ires = device.DeviceArray(1, np.int64)
+        hx, hy = host_rand(dt, hn, 83), host_rand(dt, hn, 84)
+        dot = {"s": "coral_sdot", "d": "coral_ddot", "z": "coral_zdotc"}[p]
+        ms = timed_steps(lambda: call(dot, n, x.ptr, 1, y.ptr, 1, res.ptr, None), 20, 3)
+        pdot = {"s": lambda: port.sdot(hn, hx, 1, hy, 1), "d": lambda: port.ddot(hn, hx, 1, hy, 1),
+                "z": lambda: port.zdotc(hn, hx, 1, hy, 1)}[p]
+        emit(f"{dot[6:].upper()} n=2^27", ms, 2 * es * n, cpu_leg(pdot, 2 * es * hn, "GB/s", 1e9, f"n=2^22"))
+        ms = timed_steps(lambda: call(f"coral_{p}axpy", n, scal(dt, 0.5), x.ptr, 1, y.ptr, 1, None), 20, 3)
+        hy2 = hy.copy()
+        emit(f"{p.upper()}AXPY n=2^27", ms, 3 * es * n,
+             cpu_leg(lambda: getattr(port, f"{p}axpy")(hn, 0.5, hx, 1, hy2, 1), 3 * es * hn, "GB/s", 1e9, "n=2^22"))
+        nrm = {"s": "snrm2", "d": "dnrm2", "z": "dznrm2"}[p]
+        ms = timed_steps(lambda: call(f"coral_{nrm}", n, x.ptr, 1, res.ptr, None), 20, 3)
+        emit(f"{nrm.upper()} n=2^27", ms, es * n,
+             cpu_leg(lambda: getattr(port, nrm)(hn, hx, 1), es * hn, "GB/s", 1e9, "n=2^22"))
+        ms = timed_steps(lambda: call(f"coral_i{p}amax", n, x.ptr, 1, ires.ptr, None), 20, 3)
+        emit(f"I{p.upper()}AMAX n=2^27", ms, es * n,
+             cpu_leg(lambda: getattr(port, f"i{p}amax")(hn, hx, 1), es * hn, "GB/s", 1e9, "n=2^22"))
+        ms = timed_steps(lambda: call(f"coral_{p}scal", n, scal(dt, 1.0), x.ptr, 1, None), 20, 3)
+        emit(f"{p.upper()}SCAL n=2^27", ms, 2 * es * n)
+        del x, y
+
+
 def run_suite(which):
     from coral_amd._lib import load
 
     lib = load()
     if lib.coral_init() != 0:
         raise SystemExit(f"coral_init failed: {lib.coral_last_error().decode()}")
-    table = {"c1": suite_c1, "c2": suite_c2, "c3": suite_c3, "c4": suite_c4, "c5": suite_c5}
+    table = {"c1": suite_c1, "c2": suite_c2, "c3": suite_c3, "c4": suite_c4, "c5": suite_c5, "rows": suite_rows}
     for w in which:
         log(f"suite {w} ...")
         table[w]()
@@ -537,7 +674,8 @@ def main():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--gemv-n", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--suite", default="", help="comma list of c1,c2,c3,c4,c5: one JSON line per configuration")
+    ap.add_argument("--suite", default="",
+                    help="comma list of c1,c2,c3,c4,c5,rows: one JSON line per configuration")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:

@@ -14,6 +14,59 @@ constexpr int kThreads = 256;
 
 static int64_t reduce_grid(int64_t n) { return std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kThreads * 8), 1024)); }
 
+typedef uint32_t l1v4u __attribute__((ext_vector_type(4)));
+
+// Visits elements [0, n) of x (and of y when Y) in a per-thread order that depends
+// only on n and the grid.  Contiguous 16-byte-aligned inputs are read with 16-byte
+// nontemporal loads, kU vectors in flight per thread, each wave instruction covering
+// one contiguous 1 KiB run; other inputs one element per thread per step.  Each
+// thread visits its elements in increasing index order.  f(i, x_i, y_i).
+constexpr int kU = 4;
+template <typename E, bool Y, class F>
+__device__ __forceinline__ void stream_visit(int64_t n, const E *__restrict__ x, int64_t incx,
+                                             const E *__restrict__ y, int64_t incy, bool vec, F &&f) {
+  constexpr int EV = 16 / sizeof(E);
+  const int64_t t = threadIdx.x, nt = (int64_t)gridDim.x * kThreads;
+  int64_t done = 0;
+  if (vec) {
+    const int64_t nv = n / EV;  // whole vectors
+    const l1v4u *xv = reinterpret_cast<const l1v4u *>(x);
+    const l1v4u *yv = reinterpret_cast<const l1v4u *>(y);
+    for (int64_t base = (int64_t)blockIdx.x * kThreads * kU; base < nv; base += nt * kU) {
+      l1v4u a[kU], b[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int64_t v = base + u * kThreads + t;
+        if (v < nv) {
+          a[u] = __builtin_nontemporal_load(xv + v);
+          if constexpr (Y) b[u] = __builtin_nontemporal_load(yv + v);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int64_t v = base + u * kThreads + t;
+        if (v < nv) {
+          E ea[EV], eb[EV];
+          __builtin_memcpy(ea, &a[u], 16);
+          if constexpr (Y) __builtin_memcpy(eb, &b[u], 16);
+#pragma unroll
+          for (int e = 0; e < EV; ++e) f(v * EV + e, ea[e], Y ? eb[e] : ea[e]);
+        }
+      }
+    }
+    done = nv * EV;
+  }
+  for (int64_t i = done + (int64_t)blockIdx.x * kThreads + t; i < n; i += nt) {
+    const E a = x[i * incx];
+    f(i, a, Y ? y[i * incy] : a);
+  }
+}
+
+template <typename E>
+static bool contiguous16(const E *p, int64_t inc) {
+  return inc == 1 && ((uintptr_t)p % 16 == 0);
+}
+
 // block-wide sum of one value per thread (256 threads = 4 wavefronts)
 template <typename A>
 __device__ __forceinline__ A block_sum(A v, A *sh) {
@@ -34,11 +87,11 @@ __device__ __forceinline__ A block_sum(A v, A *sh) {
 // ---------------------------------------------------------------------------
 template <typename E, bool CONJ>
 __global__ __launch_bounds__(kThreads) void dot_partial_kernel(int64_t n, const E *__restrict__ x, int64_t incx,
-                                                              const E *__restrict__ y, int64_t incy, E *part) {
+                                                              const E *__restrict__ y, int64_t incy, E *part,
+                                                              int vec) {
   __shared__ E sh[4];
   E acc = ezero<E>();
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
-    acc = emad<CONJ>(acc, x[i * incx], y[i * incy]);
+  stream_visit<E, true>(n, x, incx, y, incy, vec, [&](int64_t, E a, E b) { acc = emad<CONJ>(acc, a, b); });
   E r = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
@@ -75,7 +128,7 @@ static int dot_impl(int64_t n, const E *x, int64_t incx, const E *y, int64_t inc
   void *part = nullptr;
   CORAL_TRY(workspace(s, WS_SMALL, sizeof(E) * (size_t)g, &part));
   hipLaunchKernelGGL((dot_partial_kernel<E, CONJ>), dim3((unsigned)g), dim3(kThreads), 0, s, n, x, incx, y, incy,
-                     (E *)part);
+                     (E *)part, (int)(contiguous16(x, incx) && contiguous16(y, incy)));
   CORAL_LAUNCH_CHECK();
   hipLaunchKernelGGL(sum_final_kernel<E>, dim3(1), dim3(kThreads), 0, s, (const E *)part, g, result);
   CORAL_LAUNCH_CHECK();
@@ -95,6 +148,38 @@ __global__ __launch_bounds__(kThreads) void axpy_kernel(int64_t n, E alpha, cons
   }
 }
 
+// contiguous, 16-byte aligned x and y: kU vectors of each in flight per thread
+template <typename E>
+__global__ __launch_bounds__(kThreads) void axpy_vec_kernel(int64_t n, E alpha, const E *__restrict__ x,
+                                                           E *__restrict__ y) {
+  constexpr int EV = 16 / sizeof(E);
+  const int64_t nv = n / EV, nt = (int64_t)gridDim.x * kThreads;
+  const l1v4u *xv = reinterpret_cast<const l1v4u *>(x);
+  l1v4u *yv = reinterpret_cast<l1v4u *>(y);
+  for (int64_t base = (int64_t)blockIdx.x * kThreads * kU; base < nv; base += nt * kU) {
+    l1v4u a[kU], b[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t v = base + u * kThreads + threadIdx.x;
+      if (v < nv) { a[u] = __builtin_nontemporal_load(xv + v); b[u] = yv[v]; }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t v = base + u * kThreads + threadIdx.x;
+      if (v < nv) {
+        E ea[EV], eb[EV];
+        __builtin_memcpy(ea, &a[u], 16);
+        __builtin_memcpy(eb, &b[u], 16);
+#pragma unroll
+        for (int e = 0; e < EV; ++e) eb[e] = emad(eb[e], alpha, ea[e]);
+        __builtin_memcpy(&b[u], eb, 16);
+        yv[v] = b[u];
+      }
+    }
+  }
+  for (int64_t i = nv * EV + (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += nt) y[i] = emad(y[i], alpha, x[i]);
+}
+
 template <typename E>
 static int axpy_impl(int64_t n, E alpha, const E *x, int64_t incx, E *y, int64_t incy, coral_stream_t stream) {
   // n=1 alpha=2 x=3 incx=4 y=5 incy=6
@@ -105,6 +190,12 @@ static int axpy_impl(int64_t n, E alpha, const E *x, int64_t incx, E *y, int64_t
   if (!x) return -3;
   if (!y) return -5;
   CORAL_TRY(ensure_init());
+  if (contiguous16(x, incx) && contiguous16((const E *)y, incy)) {
+    const int64_t g = std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kThreads * 16), 2048));
+    hipLaunchKernelGGL(axpy_vec_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, resolve(stream), n, alpha, x, y);
+    CORAL_LAUNCH_CHECK();
+    return CORAL_OK;
+  }
   const int64_t g = std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kThreads * 4), 8192));
   hipLaunchKernelGGL(axpy_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, resolve(stream), n, alpha, x, incx, y,
                      incy);
@@ -125,12 +216,46 @@ __global__ __launch_bounds__(kThreads) void scal_kernel(int64_t n, E alpha, E *x
 }
 
 template <typename E>
+__global__ __launch_bounds__(kThreads) void scal_vec_kernel(int64_t n, E alpha, E *__restrict__ x) {
+  constexpr int EV = 16 / sizeof(E);
+  const int64_t nv = n / EV, nt = (int64_t)gridDim.x * kThreads;
+  l1v4u *xv = reinterpret_cast<l1v4u *>(x);
+  for (int64_t base = (int64_t)blockIdx.x * kThreads * kU; base < nv; base += nt * kU) {
+    l1v4u a[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t v = base + u * kThreads + threadIdx.x;
+      if (v < nv) a[u] = xv[v];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t v = base + u * kThreads + threadIdx.x;
+      if (v < nv) {
+        E ea[EV];
+        __builtin_memcpy(ea, &a[u], 16);
+#pragma unroll
+        for (int e = 0; e < EV; ++e) ea[e] = emul(ea[e], alpha);
+        __builtin_memcpy(&a[u], ea, 16);
+        xv[v] = a[u];
+      }
+    }
+  }
+  for (int64_t i = nv * EV + (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += nt) x[i] = emul(x[i], alpha);
+}
+
+template <typename E>
 static int scal_impl(int64_t n, E alpha, E *x, int64_t incx, coral_stream_t stream) {
   if (n < 0) return -1;
   if (incx <= 0) return -4;
   if (n == 0) return CORAL_OK;
   if (!x) return -3;
   CORAL_TRY(ensure_init());
+  if (contiguous16((const E *)x, incx)) {
+    const int64_t g = std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kThreads * 16), 2048));
+    hipLaunchKernelGGL(scal_vec_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, resolve(stream), n, alpha, x);
+    CORAL_LAUNCH_CHECK();
+    return CORAL_OK;
+  }
   const int64_t g = std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kThreads * 4), 8192));
   hipLaunchKernelGGL(scal_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, resolve(stream), n, alpha, x, incx);
   CORAL_LAUNCH_CHECK();
@@ -183,11 +308,10 @@ __device__ double blue_finish(Blue b) {
 
 template <typename E>
 __global__ __launch_bounds__(kThreads) void nrm2_partial_kernel(int64_t n, const E *__restrict__ x, int64_t incx,
-                                                               Blue *part) {
+                                                               Blue *part, int vec) {
   __shared__ Blue sh[kThreads];
   Blue b{0, 0, 0};
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-    E v = x[i * incx];
+  stream_visit<E, false>(n, x, incx, x, incx, vec, [&](int64_t, E v, E) {
     if constexpr (sizeof(typename elem_traits<E>::real) == 4) {  // binary32: plain double sum of squares
       if constexpr (elem_traits<E>::cplx) { b.med = fma((double)v.x, (double)v.x, b.med); b.med = fma((double)v.y, (double)v.y, b.med); }
       else b.med = fma((double)v, (double)v, b.med);
@@ -195,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void nrm2_partial_kernel(int64_t n, const
       if constexpr (elem_traits<E>::cplx) { blue_acc(b, v.x); blue_acc(b, v.y); }
       else blue_acc(b, v);
     }
-  }
+  });
   sh[threadIdx.x] = b;
   __syncthreads();
   for (int w = kThreads / 2; w > 0; w >>= 1) {
@@ -236,7 +360,8 @@ static int nrm2_impl(int64_t n, const E *x, int64_t incx, R *result, coral_strea
   const int64_t g = reduce_grid(n);
   void *part = nullptr;
   CORAL_TRY(workspace(s, WS_SMALL, sizeof(Blue) * (size_t)g, &part));
-  hipLaunchKernelGGL(nrm2_partial_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, s, n, x, incx, (Blue *)part);
+  hipLaunchKernelGGL(nrm2_partial_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, s, n, x, incx, (Blue *)part,
+                     (int)contiguous16(x, incx));
   CORAL_LAUNCH_CHECK();
   hipLaunchKernelGGL(nrm2_final_kernel<R>, dim3(1), dim3(kThreads), 0, s, (const Blue *)part, g, result,
                      (int)(sizeof(R) == 4));
@@ -270,13 +395,15 @@ __device__ __forceinline__ double mag(E v) {
 
 template <typename E>
 __global__ __launch_bounds__(kThreads) void iamax_partial_kernel(int64_t n, const E *__restrict__ x, int64_t incx,
-                                                                ValIdx *part) {
+                                                                ValIdx *part, int vec) {
   __shared__ ValIdx sh[kThreads];
   ValIdx b{-1.0, INT64_MAX};  // -1: nothing seen (magnitudes are >= 0)
-  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-    double m = mag(x[i * incx]);
-    if (m == m && m > b.v) { b.v = m; b.i = i; }  // strict '>' keeps the first index per thread
-  }
+  // each thread visits its indices in increasing order, so the strict '>' keeps the
+  // first index of a maximum per thread
+  stream_visit<E, false>(n, x, incx, x, incx, vec, [&](int64_t i, E v, E) {
+    const double m = mag(v);
+    if (m == m && m > b.v) { b.v = m; b.i = i; }
+  });
   sh[threadIdx.x] = b;
   __syncthreads();
   for (int w = kThreads / 2; w > 0; w >>= 1) {
@@ -315,7 +442,8 @@ static int iamax_impl(int64_t n, const E *x, int64_t incx, int64_t *result, cora
   const int64_t g = reduce_grid(n);
   void *part = nullptr;
   CORAL_TRY(workspace(s, WS_SMALL, sizeof(ValIdx) * (size_t)g, &part));
-  hipLaunchKernelGGL(iamax_partial_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, s, n, x, incx, (ValIdx *)part);
+  hipLaunchKernelGGL(iamax_partial_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, s, n, x, incx, (ValIdx *)part,
+                     (int)contiguous16(x, incx));
   CORAL_LAUNCH_CHECK();
   hipLaunchKernelGGL(iamax_final_kernel, dim3(1), dim3(kThreads), 0, s, (const ValIdx *)part, g, result);
   CORAL_LAUNCH_CHECK();
