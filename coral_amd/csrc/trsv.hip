@@ -42,16 +42,28 @@ struct TriView {
     E v = A[rc + ri * lda];
     return trans == 2 ? econj(v) : v;
   }
+  // M(i, c) when the layout is known at compile time (RC: trans == 0)
+  template <bool RC>
+  __device__ __forceinline__ E get_t(int64_t i, int64_t c) const {
+    const int64_t ri = map(i), rc = map(c);
+    if constexpr (RC) {
+      return A[ri + rc * lda];
+    } else {
+      E v = A[rc + ri * lda];
+      return trans == 2 ? econj(v) : v;
+    }
+  }
   // true when consecutive i are adjacent in memory (coalesced thread-per-row loads)
   __device__ __forceinline__ bool row_contig() const { return trans == 0; }
 };
 
 // Loads tile (rows i0.., cols c0..) of M into registers, 16 elements per thread.
 // diag: only the strictly lower part (c < i) is read; the rest is zero.
-template <typename E>
+// RCT: -1 decides the layout at run time; 0 / 1 fix it at compile time (1: trans == 0).
+template <int RCT = -1, typename E>
 __device__ __forceinline__ void load_tile(const TriView<E> &v, int64_t i0, int64_t c0, bool diag, E (&reg)[16]) {
   const int t = threadIdx.x;
-  const bool rc = v.row_contig();
+  const bool rc = RCT < 0 ? v.row_contig() : (RCT == 1);
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     // row-contiguous: lanes walk rows; otherwise lanes walk columns
@@ -59,14 +71,15 @@ __device__ __forceinline__ void load_tile(const TriView<E> &v, int64_t i0, int64
     const int c = rc ? ((t >> 6) + 4 * k) : (t & 63);
     const int64_t i = i0 + r, cc = c0 + c;
     const bool ok = i < v.n && cc < v.n && (!diag || c < r);
-    reg[k] = ok ? v.get(i, cc) : ezero<E>();
+    if constexpr (RCT < 0) reg[k] = ok ? v.get(i, cc) : ezero<E>();
+    else reg[k] = ok ? v.template get_t<RCT == 1>(i, cc) : ezero<E>();
   }
 }
 
-template <typename E>
+template <int RCT = -1, typename E>
 __device__ __forceinline__ void store_tile(const TriView<E> &v, E (*T)[NB + 1], const E (&reg)[16]) {
   const int t = threadIdx.x;
-  const bool rc = v.row_contig();
+  const bool rc = RCT < 0 ? v.row_contig() : (RCT == 1);
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const int r = rc ? (t & 63) : ((t >> 6) + 4 * k);
@@ -199,7 +212,8 @@ __global__ void stage_init_kernel(W *st, int64_t nwords, W sentinel, int *ctrl) 
 }
 
 // Persistent forward solve of the virtual lower-triangular M (see the file comment).
-template <typename E>
+// RC: the layout of M (1: trans == 0, lanes walk rows; 0: lanes walk columns).
+template <typename E, int RC>
 __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t incx, int unit,
                                                    typename Stage<E>::W *stage, int *ctrl, int nb) {
   __shared__ E T[NB][NB + 1];
@@ -220,8 +234,8 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
   // solve on the critical path is one triangular matrix-vector product.
   {
     E dreg[16];
-    load_tile(v, i0, i0, true, dreg);
-    store_tile(v, Td, dreg);
+    load_tile<RC>(v, i0, i0, true, dreg);
+    store_tile<RC>(v, Td, dreg);
     __syncthreads();
     const int64_t i = i0 + lane;
     const E d = (unit || i >= v.n) ? eone<E>() : v.get(i, i);
@@ -265,7 +279,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
   E reg0[16], reg1[16];
   auto step = [&](int j, E(&cur)[16]) {
     __syncthreads();  // every wavefront is done reading T and xs of tile j-1
-    store_tile(v, T, cur);
+    store_tile<RC>(v, T, cur);
     if (q == 0) {
       E val = xnext;
       if (xnext_j != j) {
@@ -286,15 +300,15 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
       if (j + 1 < b && load_block(j + 1, xnext)) xnext_j = j + 1;
     }
     __syncthreads();
-    if (j + 2 < b) load_tile(v, i0, (int64_t)(j + 2) * NB, false, cur);
+    if (j + 2 < b) load_tile<RC>(v, i0, (int64_t)(j + 2) * NB, false, cur);
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int c = q * 16 + k;
       acc = emad(acc, T[c][lane], xs[c]);
     }
   };
-  if (b > 0) load_tile(v, i0, 0, false, reg0);
-  if (b > 1) load_tile(v, i0, NB, false, reg1);
+  if (b > 0) load_tile<RC>(v, i0, 0, false, reg0);
+  if (b > 1) load_tile<RC>(v, i0, NB, false, reg1);
   for (int j = 0; j < b; j += 2) {
     step(j, reg0);
     if (j + 1 < b) step(j + 1, reg1);
@@ -430,8 +444,12 @@ static int trsv_impl(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const
                      (W *)stage, nwords, Stage<E>::kSentinel, (int *)ctrl);
   CORAL_LAUNCH_CHECK();
   TriView<E> v = make_view(uplo, trans, a, lda, n);
-  hipLaunchKernelGGL(trsv_kernel<E>, dim3((unsigned)nb), dim3(256), 0, s, v, x, incx,
-                     (int)(diag == CORAL_UNIT_DIAGONAL), (W *)stage, (int *)ctrl, nb);
+  if (v.trans == 0)
+    hipLaunchKernelGGL((trsv_kernel<E, 1>), dim3((unsigned)nb), dim3(256), 0, s, v, x, incx,
+                       (int)(diag == CORAL_UNIT_DIAGONAL), (W *)stage, (int *)ctrl, nb);
+  else
+    hipLaunchKernelGGL((trsv_kernel<E, 0>), dim3((unsigned)nb), dim3(256), 0, s, v, x, incx,
+                       (int)(diag == CORAL_UNIT_DIAGONAL), (W *)stage, (int *)ctrl, nb);
   CORAL_LAUNCH_CHECK();
   return CORAL_OK;
 }
