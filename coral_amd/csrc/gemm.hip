@@ -686,6 +686,64 @@ __global__ __launch_bounds__(64 * WVM * WVN) void gemm_dma_kernel(GemmArgs g) {
   epilogue<TR, EPI, FM, FN>(acc, g, (int64_t)tm * BM + wm * WM, (int64_t)tn * BN + wn * WN, lane, z);
 }
 
+// Small real problems (fewer 64x64 output tiles than CUs): one workgroup per 32x32
+// tile of C reads op(A) and op(B) straight from the caller's arrays (coalesced along
+// whichever index is contiguous), writes them to LDS in fragment order and runs the
+// MFMAs; four wavefronts each own a 16x16 block.  No pack pass: one launch.  The next
+// K step is loaded into registers while the current one is computed.
+template <class TR>
+__global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, const typename TR::T *__restrict__ A,
+                                                         int64_t asr, int64_t ask, const typename TR::T *__restrict__ B,
+                                                         int64_t bsr, int64_t bsk, int64_t K) {
+  using T = typename TR::T;
+  using acc_t = typename TR::acc_t;
+  constexpr int VE = TR::VE, KF = TR::KF, SM = 32, BK = 32, KC = BK / KF;
+  constexpr int FV = KC * (SM / 16) * 64;  // fragment vectors per operand tile
+  constexpr int EPT = SM * BK / 256;       // elements per thread per operand
+  static_assert(!TR::SPLIT && TR::PL == 1, "plain MFMA types only");
+  __shared__ v4u la[FV], lb[FV];
+  T *las = reinterpret_cast<T *>(la);
+  T *lbs = reinterpret_cast<T *>(lb);
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, wm = wave & 1, wn = wave >> 1;
+  const int64_t m0 = (int64_t)blockIdx.x * SM, n0 = (int64_t)blockIdx.y * SM;
+  T va[EPT], vb[EPT];
+  auto gather = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+      const int e = t + 256 * i;
+      int r = asr == 1 ? e % SM : e / BK, kk = asr == 1 ? e / SM : e % BK;
+      va[i] = (m0 + r < g.M && k0 + kk < K) ? A[(m0 + r) * asr + (k0 + kk) * ask] : T(0);
+      r = bsr == 1 ? e % SM : e / BK;
+      kk = bsr == 1 ? e / SM : e % BK;
+      vb[i] = (n0 + r < g.N && k0 + kk < K) ? B[(n0 + r) * bsr + (k0 + kk) * bsk] : T(0);
+    }
+  };
+  auto frag_pos = [](int r, int kk) {  // element offset of (row r, k kk) in the fragment-ordered tile
+    return (((kk / KF) * (SM / 16) + r / 16) * 64 + (r % 16) + 16 * ((kk % KF) / VE)) * VE + kk % VE;
+  };
+  acc_t acc = acc_t{};
+  gather(0);
+  for (int64_t k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();  // the previous step's fragments are consumed
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+      const int e = t + 256 * i;
+      las[asr == 1 ? frag_pos(e % SM, e / SM) : frag_pos(e / BK, e % BK)] = va[i];
+      lbs[bsr == 1 ? frag_pos(e % SM, e / SM) : frag_pos(e / BK, e % BK)] = vb[i];
+    }
+    __syncthreads();
+    if (k0 + BK < K) gather(k0 + BK);
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) {
+      const v4u fa = la[(kc * (SM / 16) + wm) * 64 + lane];
+      const v4u fb = lb[(kc * (SM / 16) + wn) * 64 + lane];
+      TR::mma(acc, &fa, &fb);
+    }
+  }
+  const acc_t accs[1][1] = {{acc}};
+  epilogue<TR, EPI_REAL, 1, 1>(accs, g, m0 + wm * 16, n0 + wn * 16, lane, 0);
+}
+
 // C := beta*C for the quick return (alpha == 0 or k == 0), never reading C when beta == 0.
 template <typename T, bool CPLX>
 __global__ void scale_c_kernel(T *C, int64_t m, int64_t n, int64_t ldc, double br, double bi) {
@@ -1056,11 +1114,30 @@ static int real_gemm(const char *routine, int32_t op_a, int32_t op_b, int64_t m,
   hipStream_t s = resolve(stream);
   if (trivial) return quick_scale<T, false>(c, m, n, ldc, (double)beta, 0.0, s);
   int tile = tuned_tile(routine, op_a, op_b, m, n, k);
-  if (tile < 0 || tile >= Menu<TR>::n) tile = default_tile<TR>(m, n, k);
-  const TileShape ts = Menu<TR>::t()[tile];
   FetchReal<T> fa{a, 0, 0}, fb{b, 0, 0};
   strides_a<T>(op_a, lda, fa.sr, fa.sk);
   strides_b<T>(op_b, ldb, fb.sr, fb.sk);
+  if constexpr (!TR::SPLIT) {
+    // small problems: too few 64x64 tiles to fill the chip; one unpacked launch
+    if (tile < 0 && cdiv(m, 64) * cdiv(n, 64) < num_cus()) {
+      GemmArgs g{};
+      g.M = m;
+      g.N = n;
+      g.C = c;
+      g.ldc = ldc;
+      g.ar = (double)alpha;
+      g.br = (double)beta;
+      g.beta_zero = beta == T(0);
+      const int slot = timing_begin(TK_GEMM, s);
+      hipLaunchKernelGGL(gemm_small_kernel<TR>, dim3((unsigned)cdiv(m, 32), (unsigned)cdiv(n, 32)), dim3(256), 0, s,
+                         g, a, fa.sr, fa.sk, b, fb.sr, fb.sk, k);
+      CORAL_LAUNCH_CHECK();
+      timing_end(slot, s);
+      return CORAL_OK;
+    }
+  }
+  if (tile < 0 || tile >= Menu<TR>::n) tile = default_tile<TR>(m, n, k);
+  const TileShape ts = Menu<TR>::t()[tile];
   if constexpr (TR::SPLIT) {
     uint32_t *ex = nullptr;
     CORAL_TRY(workspace(s, WS_AUX0, sizeof(uint32_t) * (size_t)(m + n), (void **)&ex));
