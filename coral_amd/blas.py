@@ -11,7 +11,8 @@ back as ``CoralError`` (CORAL itself panics in debug builds and has undefined
 behaviour in release builds on a bad length/stride).
 
 Reference signatures: level3/sgemm.rs:106, dgemm.rs:107, cgemm.rs:127, zgemm.rs:123;
-level2/dgemv.rs:61, cgemv.rs:67, dsymv.rs:99, chemv.rs:96, dtrsv.rs:62, dtrmv.rs:58;
+level2/dgemv.rs:61, cgemv.rs:67, dsymv.rs:99, chemv.rs:96, dtrsv.rs:62, dtrmv.rs:58,
+sger.rs:68, cgeru.rs:63, cgerc.rs:62, dsyr.rs:61, dsyr2.rs:63, cher.rs:62, cher2.rs:65;
 level1/sdot.rs:42, cdotc.rs:46, daxpy.rs:42, snrm2.rs:45, isamax.rs:43 (all under
 coral-aarch64/src/).
 """
@@ -141,6 +142,47 @@ def _tr(p, what):
 strsv, dtrsv, ctrsv, ztrsv = (_tr(p, "trsv") for p in "sdcz")
 strmv, dtrmv, ctrmv, ztrmv = (_tr(p, "trmv") for p in "sdcz")
 
+
+def _ger(p, name):
+    cplx = p in "cz"
+
+    def f(n_rows, n_cols, alpha, x, incx, y, incy, matrix, lda):
+        al = _scalar2(alpha, _REAL[p]) if cplx else alpha
+        call(f"coralh_{name}", n_rows, n_cols, al, _buf(x, p), incx, _buf(y, p), incy, _buf(matrix, p, True), lda)
+
+    f.__name__ = name
+    return f
+
+
+sger, dger = _ger("s", "sger"), _ger("d", "dger")
+cgeru, cgerc = _ger("c", "cgeru"), _ger("c", "cgerc")
+zgeru, zgerc = _ger("z", "zgeru"), _ger("z", "zgerc")
+
+
+def _syr(p, name):
+    def f(uplo, n, alpha, x, incx, matrix, lda):  # alpha is real, also for cher/zher
+        call(f"coralh_{name}", int(uplo), n, alpha, _buf(x, p), incx, _buf(matrix, p, True), lda)
+
+    f.__name__ = name
+    return f
+
+
+ssyr, dsyr, cher, zher = _syr("s", "ssyr"), _syr("d", "dsyr"), _syr("c", "cher"), _syr("z", "zher")
+
+
+def _syr2(p, name):
+    cplx = p in "cz"
+
+    def f(uplo, n, alpha, x, incx, y, incy, matrix, lda):
+        al = _scalar2(alpha, _REAL[p]) if cplx else alpha
+        call(f"coralh_{name}", int(uplo), n, al, _buf(x, p), incx, _buf(y, p), incy, _buf(matrix, p, True), lda)
+
+    f.__name__ = name
+    return f
+
+
+ssyr2, dsyr2 = _syr2("s", "ssyr2"), _syr2("d", "dsyr2")
+cher2, zher2 = _syr2("c", "cher2"), _syr2("z", "zher2")
 
 # ---------------------------------------------------------------- level 1
 def _dot(p, name):

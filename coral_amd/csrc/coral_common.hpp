@@ -143,6 +143,51 @@ template <typename E> __device__ __forceinline__ E eshfl_xor(E v, int off) {
   if constexpr (elem_traits<E>::cplx) { v.x = __shfl_xor(v.x, off, 64); v.y = __shfl_xor(v.y, off, 64); return v; }
   else return __shfl_xor(v, off, 64);
 }
+// CORAL's two roundings of one axpy element update y += c*x.
+//  fused: the NEON body of the unit-stride axpy (vfmaq / vfmsq):
+//         y = fma(x, c, y); complex yr = fma(cr, xr, yr); yr = fma(-ci, xi, yr);
+//         yi = fma(cr, xi, yi); yi = fma(ci, xr, yi)  (caxpy.rs:86-89)
+//  plain: its scalar tail and strided loop, two roundings per product and sum:
+//         y = y + c*x; complex yr += cr*xr - ci*xi; yi += cr*xi + ci*xr  (caxpy.rs:144-145)
+// The body covers the first floor(len/Q)*Q elements of a unit-stride call
+// (coral-aarch64/src/level1/saxpy.rs:63-112 Q = 8, daxpy.rs Q = 4, caxpy.rs:70-132 Q = 4,
+// zaxpy.rs:69-131 Q = 2).
+template <typename E> struct axpy_body;
+template <> struct axpy_body<float>   { static constexpr int64_t Q = 8; };
+template <> struct axpy_body<double>  { static constexpr int64_t Q = 4; };
+template <> struct axpy_body<float2>  { static constexpr int64_t Q = 4; };
+template <> struct axpy_body<double2> { static constexpr int64_t Q = 2; };
+
+__device__ __forceinline__ float axpy_upd(float y, float c, float x, bool fused) {
+#pragma clang fp contract(off)
+  const float f = __builtin_fmaf(x, c, y);
+  const float p = y + c * x;
+  return fused ? f : p;
+}
+__device__ __forceinline__ double axpy_upd(double y, double c, double x, bool fused) {
+#pragma clang fp contract(off)
+  const double f = __builtin_fma(x, c, y);
+  const double p = y + c * x;
+  return fused ? f : p;
+}
+template <typename E>
+__device__ __forceinline__ E axpy_upd_c(E y, E c, E x, bool fused) {
+#pragma clang fp contract(off)
+  using R = typename elem_traits<E>::real;
+  R fr = __builtin_fma(c.x, x.x, y.x);
+  fr = __builtin_fma(-c.y, x.y, fr);
+  R fi = __builtin_fma(c.x, x.y, y.y);
+  fi = __builtin_fma(c.y, x.x, fi);
+  const R pr = y.x + (c.x * x.x - c.y * x.y);
+  const R pi = y.y + (c.x * x.y + c.y * x.x);
+  E r;
+  r.x = fused ? fr : pr;
+  r.y = fused ? fi : pi;
+  return r;
+}
+__device__ __forceinline__ float2 axpy_upd(float2 y, float2 c, float2 x, bool f) { return axpy_upd_c(y, c, x, f); }
+__device__ __forceinline__ double2 axpy_upd(double2 y, double2 c, double2 x, bool f) { return axpy_upd_c(y, c, x, f); }
+
 template <typename E> __device__ __forceinline__ E wave_esum(E v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = eadd(v, eshfl_xor(v, off));

@@ -152,6 +152,23 @@ static int red_host(size_t es, int64_t n, const void *x, int64_t incx, const voi
   return finish();
 }
 
+// rank updates: vectors in, matrix in/out
+template <typename F>
+static int rank_host(size_t es, int64_t rows, int64_t cols, const void *x, int64_t incx, const void *y,
+                     int64_t incy, void *a, int64_t lda, bool sane, F call) {
+  HOST_PROLOGUE();
+  if (!sane || rows < 0 || cols < 0 || incx <= 0 || (y && incy <= 0) || lda < 1) return call(x, y, a, st);
+  void *dx, *dy, *da;
+  const size_t ab = mat_bytes(rows, cols, lda, es);
+  CORAL_TRY(put(0, x, vec_bytes(rows, incx, es), &dx));
+  CORAL_TRY(put(1, y, vec_bytes(cols, incy, es), &dy));
+  CORAL_TRY(put(2, a, ab, &da));
+  int s = call(dx ? dx : x, dy ? dy : y, da ? da : a, st);
+  if (s != CORAL_OK) { finish(); return s; }
+  CORAL_TRY(get(a, da, ab));
+  return finish();
+}
+
 }  // namespace host
 }  // namespace coral
 
@@ -224,6 +241,49 @@ TR_HOST(strmv, float, 4)
 TR_HOST(dtrmv, double, 8)
 TR_HOST(ctrmv, float, 8)
 TR_HOST(ztrmv, double, 16)
+
+// ---- rank updates ----
+#define GER_HOST(NAME, T, ES, S_T)                                                                               \
+  int coralh_##NAME(int64_t m, int64_t n, S_T alpha, const T *x, int64_t incx, const T *y, int64_t incy, T *a,  \
+                    int64_t lda) {                                                                               \
+    return rank_host(ES, m, n, x, incx, y, incy, a, lda, true,                                                   \
+                     [&](const void *dx, const void *dy, void *da, coral_stream_t st) {                         \
+                       return coral_##NAME(m, n, alpha, (const T *)dx, incx, (const T *)dy, incy, (T *)da, lda, \
+                                           st);                                                                  \
+                     });                                                                                         \
+  }
+GER_HOST(sger, float, 4, float)
+GER_HOST(dger, double, 8, double)
+GER_HOST(cgeru, float, 8, const float *)
+GER_HOST(cgerc, float, 8, const float *)
+GER_HOST(zgeru, double, 16, const double *)
+GER_HOST(zgerc, double, 16, const double *)
+
+#define SYR_HOST(NAME, T, ES, S_T)                                                                               \
+  int coralh_##NAME(int32_t uplo, int64_t n, S_T alpha, const T *x, int64_t incx, T *a, int64_t lda) {          \
+    return rank_host(ES, n, n, x, incx, nullptr, 1, a, lda, valid_uplo(uplo),                                   \
+                     [&](const void *dx, const void *, void *da, coral_stream_t st) {                           \
+                       return coral_##NAME(uplo, n, alpha, (const T *)dx, incx, (T *)da, lda, st);              \
+                     });                                                                                         \
+  }
+SYR_HOST(ssyr, float, 4, float)
+SYR_HOST(dsyr, double, 8, double)
+SYR_HOST(cher, float, 8, float)
+SYR_HOST(zher, double, 16, double)
+
+#define SYR2_HOST(NAME, T, ES, S_T)                                                                              \
+  int coralh_##NAME(int32_t uplo, int64_t n, S_T alpha, const T *x, int64_t incx, const T *y, int64_t incy,     \
+                    T *a, int64_t lda) {                                                                         \
+    return rank_host(ES, n, n, x, incx, y, incy, a, lda, valid_uplo(uplo),                                       \
+                     [&](const void *dx, const void *dy, void *da, coral_stream_t st) {                         \
+                       return coral_##NAME(uplo, n, alpha, (const T *)dx, incx, (const T *)dy, incy, (T *)da,   \
+                                           lda, st);                                                             \
+                     });                                                                                         \
+  }
+SYR2_HOST(ssyr2, float, 4, float)
+SYR2_HOST(dsyr2, double, 8, double)
+SYR2_HOST(cher2, float, 8, const float *)
+SYR2_HOST(zher2, double, 16, const double *)
 
 // ---- level 1 ----
 #define DOT_HOST(NAME, T, ES, RB)                                                                                \

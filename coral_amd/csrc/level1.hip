@@ -137,20 +137,22 @@ static int dot_impl(int64_t n, const E *x, int64_t incx, const E *y, int64_t inc
 
 // ---------------------------------------------------------------------------
 // axpy: y += alpha*x (coral-aarch64/src/level1/daxpy.rs:42-132); quick return when
-// n == 0 or alpha == 0 (:51-53)
+// n == 0 or alpha == 0 (:51-53).  Bit-identical to CORAL: element i gets the fused
+// update when i < nf (the NEON body of a unit-stride call, nf = floor(n/Q)*Q) and
+// the twice-rounded one otherwise (its tail, and every element of a strided call).
 // ---------------------------------------------------------------------------
 template <typename E>
-__global__ __launch_bounds__(kThreads) void axpy_kernel(int64_t n, E alpha, const E *__restrict__ x, int64_t incx,
-                                                       E *__restrict__ y, int64_t incy) {
+__global__ __launch_bounds__(kThreads) void axpy_kernel(int64_t n, int64_t nf, E alpha, const E *__restrict__ x,
+                                                       int64_t incx, E *__restrict__ y, int64_t incy) {
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
     E *yp = y + i * incy;
-    *yp = emad(*yp, alpha, x[i * incx]);
+    *yp = axpy_upd(*yp, alpha, x[i * incx], i < nf);
   }
 }
 
 // contiguous, 16-byte aligned x and y: kU vectors of each in flight per thread
 template <typename E>
-__global__ __launch_bounds__(kThreads) void axpy_vec_kernel(int64_t n, E alpha, const E *__restrict__ x,
+__global__ __launch_bounds__(kThreads) void axpy_vec_kernel(int64_t n, int64_t nf, E alpha, const E *__restrict__ x,
                                                            E *__restrict__ y) {
   constexpr int EV = 16 / sizeof(E);
   const int64_t nv = n / EV, nt = (int64_t)gridDim.x * kThreads;
@@ -171,13 +173,14 @@ __global__ __launch_bounds__(kThreads) void axpy_vec_kernel(int64_t n, E alpha, 
         __builtin_memcpy(ea, &a[u], 16);
         __builtin_memcpy(eb, &b[u], 16);
 #pragma unroll
-        for (int e = 0; e < EV; ++e) eb[e] = emad(eb[e], alpha, ea[e]);
+        for (int e = 0; e < EV; ++e) eb[e] = axpy_upd(eb[e], alpha, ea[e], v * EV + e < nf);
         __builtin_memcpy(&b[u], eb, 16);
         yv[v] = b[u];
       }
     }
   }
-  for (int64_t i = nv * EV + (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += nt) y[i] = emad(y[i], alpha, x[i]);
+  for (int64_t i = nv * EV + (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += nt)
+    y[i] = axpy_upd(y[i], alpha, x[i], i < nf);
 }
 
 template <typename E>
@@ -190,14 +193,15 @@ static int axpy_impl(int64_t n, E alpha, const E *x, int64_t incx, E *y, int64_t
   if (!x) return -3;
   if (!y) return -5;
   CORAL_TRY(ensure_init());
+  const int64_t nf = (incx == 1 && incy == 1) ? n / axpy_body<E>::Q * axpy_body<E>::Q : 0;
   if (contiguous16(x, incx) && contiguous16((const E *)y, incy)) {
     const int64_t g = std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kThreads * 16), 2048));
-    hipLaunchKernelGGL(axpy_vec_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, resolve(stream), n, alpha, x, y);
+    hipLaunchKernelGGL(axpy_vec_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, resolve(stream), n, nf, alpha, x, y);
     CORAL_LAUNCH_CHECK();
     return CORAL_OK;
   }
   const int64_t g = std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kThreads * 4), 8192));
-  hipLaunchKernelGGL(axpy_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, resolve(stream), n, alpha, x, incx, y,
+  hipLaunchKernelGGL(axpy_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, resolve(stream), n, nf, alpha, x, incx, y,
                      incy);
   CORAL_LAUNCH_CHECK();
   return CORAL_OK;

@@ -1,0 +1,366 @@
+// Level-2 rank updates for gfx950: ?GER / ?GERU / ?GERC, ?SYR / ?SYR2, ?HER / ?HER2.
+//
+// CORAL runs these column by column: a coefficient per column (alpha*y_j, alpha*x_j
+// or their conjugate forms), skipped when it is zero, then one or two axpys of x
+// (and y) into the stored rows of that column (coral-aarch64/src/level2/
+// sger.rs:68-132, cgeru.rs:63-144, cgerc.rs:62-143, ssyr.rs:63-181,
+// ssyr2.rs:59-197, cher.rs:62-212, cher2.rs:65-311 and the d/z twins).
+//
+// Here one launch streams the stored part of A once, read-modify-write: a
+// workgroup owns 256*EV rows x a chunk of columns (EV elements per 16-byte
+// vector), keeps its rows of x (and y) in registers and has four columns of
+// 16-byte loads in flight per thread.  Every wave instruction covers one
+// contiguous 1 KiB run of a column.
+//
+// The results are bit-identical to CORAL's: each element receives the same
+// operations in the same order.  CORAL's unit-stride axpy runs a NEON body with
+// fused multiply-adds over the first floor(len/Q)*Q elements of a segment and a
+// scalar tail `y += c*x` with two roundings (Q = 8 saxpy, 4 daxpy and caxpy,
+// 2 zaxpy: saxpy.rs:63-112, daxpy.rs:63-112, caxpy.rs:70-146, zaxpy.rs:69-145);
+// its strided axpy has two roundings throughout; the strided real rank updates
+// use `x.mul_add(c, a)` (fused) inline.  The element updates are axpy_upd
+// (coral_common.hpp); this file is compiled with -ffp-contract=off so that the
+// per-column coefficients also keep CORAL's separate roundings.
+#include "coral_common.hpp"
+
+#include <algorithm>
+
+namespace coral {
+namespace rk {
+
+typedef uint32_t rv4u __attribute__((ext_vector_type(4)));
+
+enum Kind { K_GER = 0, K_GERU = 1, K_GERC = 2, K_SYR = 3, K_SYR2 = 4, K_HER = 5, K_HER2 = 6 };
+// How an element update rounds (see the file comment).
+enum Mode { M_PLAIN = 0, M_UNIT = 1, M_FUSED = 2 };
+
+constexpr int kThreads = 256;
+constexpr int kU = 4;  // columns in flight per thread
+
+template <typename E>
+struct RankArgs {
+  int64_t m, n;       // rows and columns of A (m == n for SYR/SYR2/HER/HER2)
+  E alpha;            // HER: the real alpha in .x
+  const E *x;
+  int64_t incx;
+  const E *y;
+  int64_t incy;
+  E *a;
+  int64_t lda;
+  int64_t cpc;        // columns per workgroup
+  int lower;          // stored triangle (triangular kinds)
+  int mode1, mode2;   // rounding mode of the x update / the y update
+  int unit;           // incx == 1 && incy == 1
+  int vec_a;          // A and lda allow aligned 16-byte column vectors
+  int vec_x, vec_y;   // x / y are unit stride and 16-byte aligned
+};
+
+// number of leading elements of a segment of length len that get the fused update
+template <typename E>
+__device__ __forceinline__ int64_t fused_prefix(int mode, int64_t len) {
+  constexpr int64_t Q = axpy_body<E>::Q;
+  return mode == M_FUSED ? len : (mode == M_UNIT ? len / Q * Q : 0);
+}
+
+template <typename E> __device__ __forceinline__ bool nz(E v) { return !eiszero(v); }
+
+// Per-column coefficients and which updates apply (CORAL's skip rules).
+//  c1 multiplies x (applied when ap1), c2 multiplies y (applied when ap2);
+//  zdiag: Im(a_jj) is set to zero after the updates (HER/HER2).
+template <typename E, int K>
+__device__ __forceinline__ void column_coefs(const RankArgs<E> &p, int64_t j, E &c1, E &c2, bool &ap1, bool &ap2,
+                                             bool &zdiag) {
+  c1 = ezero<E>();
+  c2 = ezero<E>();
+  ap2 = false;
+  zdiag = false;
+  if constexpr (K == K_GER) {  // sger.rs:95 / :117: coeff = alpha * y_j
+    c1 = p.alpha * p.y[j * p.incy];
+    ap1 = nz(c1);
+  } else if constexpr (K == K_SYR) {  // ssyr.rs:92 / :143: aj = alpha * x_j
+    c1 = p.alpha * p.x[j * p.incx];
+    ap1 = nz(c1);
+  } else if constexpr (K == K_SYR2) {  // ssyr2.rs:88-89 / :155-156: aj_y = alpha*y_j, aj_x = alpha*x_j
+    c1 = p.alpha * p.y[j * p.incy];
+    c2 = p.alpha * p.x[j * p.incx];
+    if (p.unit) {
+      ap1 = nz(c1);
+      ap2 = nz(c2);
+    } else {  // one mul_add loop with both coefficients unless both are zero (ssyr2.rs:157-168)
+      ap1 = ap2 = nz(c1) || nz(c2);
+    }
+  } else if constexpr (K == K_GERU || K == K_GERC) {
+    const E yv = p.y[j * p.incy];
+    const auto ar = p.alpha.x, ai = p.alpha.y;
+    if constexpr (K == K_GERU) {  // cgeru.rs:92-95 / :120-121
+      c1.x = ar * yv.x - ai * yv.y;
+      c1.y = ar * yv.y + ai * yv.x;
+    } else {  // cgerc.rs:91-94 / :119-120
+      c1.x = ar * yv.x + ai * yv.y;
+      c1.y = -ar * yv.y + ai * yv.x;
+    }
+    ap1 = nz(c1);
+  } else if constexpr (K == K_HER) {  // cher.rs:93: aj = [alpha*xr, -alpha*xi]
+    const E xv = p.x[j * p.incx];
+    c1.x = p.alpha.x * xv.x;
+    c1.y = -p.alpha.x * xv.y;
+    ap1 = nz(c1);
+    zdiag = ap1;
+  } else {  // K_HER2, cher2.rs:106-109
+    const E xv = p.x[j * p.incx], yv = p.y[j * p.incy];
+    const auto ar = p.alpha.x, ai = p.alpha.y;
+    c1.x = ar * yv.x + ai * yv.y;
+    c1.y = -ar * yv.y + ai * yv.x;
+    c2.x = ar * xv.x - ai * xv.y;
+    c2.y = -ar * xv.y - ai * xv.x;
+    ap1 = nz(c1);
+    ap2 = nz(c2);
+    // unit strides zero the diagonal's imaginary part for every column
+    // (cher2.rs:136), strided calls only for updated columns (cher2.rs:215-244)
+    zdiag = p.unit || ap1 || ap2;
+  }
+}
+
+// Rows [r, r+EV) of a source vector into registers (zero beyond m).
+template <typename E>
+__device__ __forceinline__ void load_src(E (&v)[16 / sizeof(E)], const E *s, int64_t inc, bool vec, int64_t r,
+                                         int64_t m) {
+  constexpr int EV = 16 / sizeof(E);
+  if (vec && r + EV <= m) {
+    const rv4u pk = *reinterpret_cast<const rv4u *>(s + r);
+    __builtin_memcpy(&v[0], &pk, 16);
+  } else {
+#pragma unroll
+    for (int e = 0; e < EV; ++e) v[e] = (r + e < m) ? s[(r + e) * inc] : ezero<E>();
+  }
+}
+
+template <typename E, int K>
+__global__ __launch_bounds__(kThreads) void rank_kernel(RankArgs<E> p) {
+  constexpr int EV = 16 / sizeof(E);
+  constexpr bool TRI = K == K_SYR || K == K_SYR2 || K == K_HER || K == K_HER2;
+  constexpr bool TWO = K == K_SYR2 || K == K_HER2;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t r0 = (int64_t)blockIdx.x * (kThreads * EV);
+  const int64_t c0 = (int64_t)blockIdx.y * p.cpc;
+  const int64_t c1 = std::min<int64_t>(c0 + p.cpc, p.n);
+  if (TRI) {  // workgroup entirely outside the stored triangle
+    if (!p.lower && r0 > c1 - 1) return;
+    if (p.lower && r0 + kThreads * EV - 1 < c0) return;
+  }
+  const int64_t rw = r0 + (int64_t)w * 64 * EV;  // first row of this wave
+  if (rw >= p.m) return;                          // uniform per wave; no barrier below
+  const int64_t rr = rw + (int64_t)lane * EV;     // first row of this thread
+
+  E xs[EV], ys[EV];
+  load_src<E>(xs, p.x, p.incx, p.vec_x, rr, p.m);
+  if (TWO) load_src<E>(ys, p.y, p.incy, p.vec_y, rr, p.m);
+
+  for (int64_t j0 = c0; j0 < c1; j0 += kU) {
+    E av[kU][EV];
+    E cf1[kU], cf2[kU];
+    bool a1[kU], a2[kU], zd[kU], full[kU];
+    int64_t lo[kU], hi[kU], seg0[kU], nf1[kU], nf2[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t j = j0 + u;
+      a1[u] = a2[u] = zd[u] = full[u] = false;
+      lo[u] = hi[u] = seg0[u] = nf1[u] = nf2[u] = 0;
+      if (j >= c1) continue;
+      column_coefs<E, K>(p, j, cf1[u], cf2[u], a1[u], a2[u], zd[u]);
+      if (!(a1[u] || a2[u] || zd[u])) continue;
+      // stored rows of column j and the axpy segment CORAL runs over them
+      int64_t l = 0, h = p.m;
+      if (TRI) {
+        l = p.lower ? j : 0;
+        h = p.lower ? p.n : j + 1;
+      }
+      seg0[u] = l;
+      nf1[u] = fused_prefix<E>(p.mode1, h - l);
+      nf2[u] = fused_prefix<E>(p.mode2, h - l);
+      if (!(a1[u] || a2[u])) {  // only the diagonal changes (HER2, zero coefficients)
+        l = j;
+        h = j + 1;
+      }
+      if (rw >= h || rw + 64 * EV <= l) {  // this wave holds none of the rows (uniform)
+        a1[u] = a2[u] = zd[u] = false;
+        continue;
+      }
+      lo[u] = l;
+      hi[u] = h;
+      E *col = p.a + j * p.lda;
+      full[u] = p.vec_a && rr >= l && rr + EV <= h;
+      if (full[u]) {
+        const rv4u pk = __builtin_nontemporal_load(reinterpret_cast<const rv4u *>(col + rr));
+        __builtin_memcpy(&av[u][0], &pk, 16);
+      } else {
+#pragma unroll
+        for (int e = 0; e < EV; ++e) av[u][e] = (rr + e >= l && rr + e < h) ? col[rr + e] : ezero<E>();
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t j = j0 + u;
+      if (!(a1[u] || a2[u] || zd[u])) continue;
+#pragma unroll
+      for (int e = 0; e < EV; ++e) {
+        const int64_t i = rr + e, t = i - seg0[u];
+        E v = av[u][e];
+        if (a1[u]) v = axpy_upd(v, cf1[u], xs[e], t < nf1[u]);
+        if (TWO && a2[u]) v = axpy_upd(v, cf2[u], ys[e], t < nf2[u]);
+        if constexpr (elem_traits<E>::cplx) {
+          if (zd[u] && i == j) v.y = 0;
+        }
+        av[u][e] = v;
+      }
+      E *col = p.a + j * p.lda;
+      if (full[u]) {
+        rv4u pk;
+        __builtin_memcpy(&pk, &av[u][0], 16);
+        *reinterpret_cast<rv4u *>(col + rr) = pk;
+      } else {
+#pragma unroll
+        for (int e = 0; e < EV; ++e)
+          if (rr + e >= lo[u] && rr + e < hi[u]) col[rr + e] = av[u][e];
+      }
+    }
+  }
+}
+
+// CORAL argument positions of the checked arguments, per entry point.
+struct ArgPos { int uplo, m, n, alpha, x, incx, y, incy, a, lda; };
+
+template <typename E, int K>
+int rank_impl(const ArgPos &ap, int32_t uplo, int64_t m, int64_t n, E alpha, const E *x, int64_t incx, const E *y,
+              int64_t incy, E *a, int64_t lda, coral_stream_t stream) {
+  constexpr bool TRI = K == K_SYR || K == K_SYR2 || K == K_HER || K == K_HER2;
+  constexpr bool HASY = !(K == K_SYR || K == K_HER);
+  if (TRI && !valid_uplo(uplo)) return -ap.uplo;
+  if (TRI) m = n;
+  if (m < 0) return -ap.m;
+  if (n < 0) return -ap.n;
+  if (incx <= 0) return -ap.incx;
+  if (HASY && incy <= 0) return -ap.incy;
+  if (lda < imax64(1, m)) return -ap.lda;
+  // quick returns: sger.rs:80, ssyr.rs:73, ssyr2.rs:70, cher.rs:72, cher2.rs:77
+  if (m == 0 || n == 0 || eiszero(alpha)) return CORAL_OK;
+  if (!x) return -ap.x;
+  if (HASY && !y) return -ap.y;
+  if (!a) return -ap.a;
+  CORAL_TRY(ensure_init());
+  hipStream_t s = resolve(stream);
+
+  RankArgs<E> p;
+  p.m = m;
+  p.n = n;
+  p.alpha = alpha;
+  p.x = x;
+  p.incx = incx;
+  p.y = HASY ? y : x;
+  p.incy = HASY ? incy : incx;
+  p.a = a;
+  p.lda = lda;
+  p.lower = uplo == CORAL_LOWER_TRIANGULAR;
+  const bool unit = incx == 1 && (!HASY || incy == 1);
+  p.unit = unit;
+  // rounding modes, per CORAL's dispatch on the strides
+  switch (K) {
+    case K_GER: p.mode1 = p.mode2 = unit ? M_UNIT : M_FUSED; break;   // sger.rs:92 / :125
+    case K_GERU:
+    case K_GERC: p.mode1 = p.mode2 = unit ? M_UNIT : M_PLAIN; break;  // cgerc.rs:86 / :131-135
+    case K_SYR: p.mode1 = p.mode2 = incx == 1 ? M_UNIT : M_FUSED; break;  // ssyr.rs:86 / :152
+    case K_SYR2: p.mode1 = p.mode2 = unit ? M_UNIT : M_FUSED; break;      // ssyr2.rs:84 / :166-167
+    case K_HER: p.mode1 = p.mode2 = incx == 1 ? M_UNIT : M_PLAIN; break;  // cher.rs:85: caxpy(x, incx, col, 1)
+    default:  // K_HER2: caxpy(x, incx, col, 1) then caxpy(y, incy, col, 1)
+      p.mode1 = incx == 1 ? M_UNIT : M_PLAIN;
+      p.mode2 = incy == 1 ? M_UNIT : M_PLAIN;
+  }
+  p.vec_a = ((uintptr_t)a % 16 == 0) && ((lda * (int64_t)sizeof(E)) % 16 == 0);
+  p.vec_x = incx == 1 && ((uintptr_t)x % 16 == 0);
+  p.vec_y = p.incy == 1 && ((uintptr_t)p.y % 16 == 0);
+
+  constexpr int EV = 16 / sizeof(E);
+  const int64_t row_blocks = cdiv(m, (int64_t)kThreads * EV);
+  p.cpc = std::max<int64_t>(32, cdiv(n, 65535));
+  const int64_t col_blocks = cdiv(n, p.cpc);
+  if (row_blocks > 0x7fffffff) return CORAL_ERR_UNSUPPORTED;
+  const int slot = timing_begin(TK_GEMV, s);
+  hipLaunchKernelGGL((rank_kernel<E, K>), dim3((unsigned)row_blocks, (unsigned)col_blocks), dim3(kThreads), 0, s, p);
+  CORAL_LAUNCH_CHECK();
+  timing_end(slot, s);
+  return CORAL_OK;
+}
+
+constexpr ArgPos kGerPos = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};   // (m, n, alpha, x, incx, y, incy, a, lda)
+constexpr ArgPos kSyrPos = {1, 2, 2, 3, 4, 5, 0, 0, 6, 7};   // (uplo, n, alpha, x, incx, a, lda)
+constexpr ArgPos kSyr2Pos = {1, 2, 2, 3, 4, 5, 6, 7, 8, 9};  // (uplo, n, alpha, x, incx, y, incy, a, lda)
+
+}  // namespace rk
+}  // namespace coral
+
+using namespace coral;
+using namespace coral::rk;
+
+extern "C" {
+
+int coral_sger(int64_t m, int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy,
+               float *a, int64_t lda, coral_stream_t stream) {
+  return rank_impl<float, K_GER>(kGerPos, 0, m, n, alpha, x, incx, y, incy, a, lda, stream);
+}
+int coral_dger(int64_t m, int64_t n, double alpha, const double *x, int64_t incx, const double *y, int64_t incy,
+               double *a, int64_t lda, coral_stream_t stream) {
+  return rank_impl<double, K_GER>(kGerPos, 0, m, n, alpha, x, incx, y, incy, a, lda, stream);
+}
+
+#define CORAL_CGER(NAME, R, E2, MK, KIND)                                                                        \
+  int coral_##NAME(int64_t m, int64_t n, const R alpha[2], const R *x, int64_t incx, const R *y, int64_t incy,  \
+                   R *a, int64_t lda, coral_stream_t stream) {                                                   \
+    if (!alpha) return -3;                                                                                       \
+    return rank_impl<E2, KIND>(kGerPos, 0, m, n, MK(alpha[0], alpha[1]), (const E2 *)x, incx, (const E2 *)y,    \
+                               incy, (E2 *)a, lda, stream);                                                      \
+  }
+CORAL_CGER(cgeru, float, float2, make_float2, K_GERU)
+CORAL_CGER(cgerc, float, float2, make_float2, K_GERC)
+CORAL_CGER(zgeru, double, double2, make_double2, K_GERU)
+CORAL_CGER(zgerc, double, double2, make_double2, K_GERC)
+
+int coral_ssyr(int32_t uplo, int64_t n, float alpha, const float *x, int64_t incx, float *a, int64_t lda,
+               coral_stream_t stream) {
+  return rank_impl<float, K_SYR>(kSyrPos, uplo, n, n, alpha, x, incx, nullptr, 1, a, lda, stream);
+}
+int coral_dsyr(int32_t uplo, int64_t n, double alpha, const double *x, int64_t incx, double *a, int64_t lda,
+               coral_stream_t stream) {
+  return rank_impl<double, K_SYR>(kSyrPos, uplo, n, n, alpha, x, incx, nullptr, 1, a, lda, stream);
+}
+int coral_ssyr2(int32_t uplo, int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy,
+                float *a, int64_t lda, coral_stream_t stream) {
+  return rank_impl<float, K_SYR2>(kSyr2Pos, uplo, n, n, alpha, x, incx, y, incy, a, lda, stream);
+}
+int coral_dsyr2(int32_t uplo, int64_t n, double alpha, const double *x, int64_t incx, const double *y,
+                int64_t incy, double *a, int64_t lda, coral_stream_t stream) {
+  return rank_impl<double, K_SYR2>(kSyr2Pos, uplo, n, n, alpha, x, incx, y, incy, a, lda, stream);
+}
+int coral_cher(int32_t uplo, int64_t n, float alpha, const float *x, int64_t incx, float *a, int64_t lda,
+               coral_stream_t stream) {
+  return rank_impl<float2, K_HER>(kSyrPos, uplo, n, n, make_float2(alpha, 0.0f), (const float2 *)x, incx, nullptr,
+                                  1, (float2 *)a, lda, stream);
+}
+int coral_zher(int32_t uplo, int64_t n, double alpha, const double *x, int64_t incx, double *a, int64_t lda,
+               coral_stream_t stream) {
+  return rank_impl<double2, K_HER>(kSyrPos, uplo, n, n, make_double2(alpha, 0.0), (const double2 *)x, incx,
+                                   nullptr, 1, (double2 *)a, lda, stream);
+}
+int coral_cher2(int32_t uplo, int64_t n, const float alpha[2], const float *x, int64_t incx, const float *y,
+                int64_t incy, float *a, int64_t lda, coral_stream_t stream) {
+  if (!alpha) return -3;
+  return rank_impl<float2, K_HER2>(kSyr2Pos, uplo, n, n, make_float2(alpha[0], alpha[1]), (const float2 *)x, incx,
+                                   (const float2 *)y, incy, (float2 *)a, lda, stream);
+}
+int coral_zher2(int32_t uplo, int64_t n, const double alpha[2], const double *x, int64_t incx, const double *y,
+                int64_t incy, double *a, int64_t lda, coral_stream_t stream) {
+  if (!alpha) return -3;
+  return rank_impl<double2, K_HER2>(kSyr2Pos, uplo, n, n, make_double2(alpha[0], alpha[1]), (const double2 *)x,
+                                    incx, (const double2 *)y, incy, (double2 *)a, lda, stream);
+}
+
+}  // extern "C"

@@ -227,6 +227,46 @@ int coral_dtrsv_batched(int32_t uplo, int32_t trans, int32_t diag, int64_t batch
                         const int32_t *n, const double *const *a, const int32_t *lda,
                         double *const *x, coral_stream_t stream);
 
+/* Rank updates.  Results are bit-identical to CORAL's: the same per-column
+ * coefficient and skip rules, and the same rounding per element (fused where
+ * CORAL's NEON axpy body fuses, two roundings in its scalar tails and strided
+ * loops).  HER/HER2 set Im(a_jj) to zero as CORAL does.
+ * coral-aarch64/src/level2/sger.rs:68 / dger.rs:69: A += alpha * x * y^T */
+int coral_sger(int64_t m, int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy,
+               float *a, int64_t lda, coral_stream_t stream);
+int coral_dger(int64_t m, int64_t n, double alpha, const double *x, int64_t incx, const double *y, int64_t incy,
+               double *a, int64_t lda, coral_stream_t stream);
+/* coral-aarch64/src/level2/cgeru.rs:63 / cgerc.rs:62 / zgeru.rs:61 / zgerc.rs:60:
+ * A += alpha * x * y^T (u) or alpha * x * y^H (c) */
+int coral_cgeru(int64_t m, int64_t n, const float alpha[2], const float *x, int64_t incx, const float *y,
+                int64_t incy, float *a, int64_t lda, coral_stream_t stream);
+int coral_cgerc(int64_t m, int64_t n, const float alpha[2], const float *x, int64_t incx, const float *y,
+                int64_t incy, float *a, int64_t lda, coral_stream_t stream);
+int coral_zgeru(int64_t m, int64_t n, const double alpha[2], const double *x, int64_t incx, const double *y,
+                int64_t incy, double *a, int64_t lda, coral_stream_t stream);
+int coral_zgerc(int64_t m, int64_t n, const double alpha[2], const double *x, int64_t incx, const double *y,
+                int64_t incy, double *a, int64_t lda, coral_stream_t stream);
+/* coral-aarch64/src/level2/ssyr.rs:63 / dsyr.rs:61: A += alpha * x * x^T (stored triangle) */
+int coral_ssyr(int32_t uplo, int64_t n, float alpha, const float *x, int64_t incx, float *a, int64_t lda,
+               coral_stream_t stream);
+int coral_dsyr(int32_t uplo, int64_t n, double alpha, const double *x, int64_t incx, double *a, int64_t lda,
+               coral_stream_t stream);
+/* coral-aarch64/src/level2/ssyr2.rs:59 / dsyr2.rs:63: A += alpha * (x * y^T + y * x^T) */
+int coral_ssyr2(int32_t uplo, int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy,
+                float *a, int64_t lda, coral_stream_t stream);
+int coral_dsyr2(int32_t uplo, int64_t n, double alpha, const double *x, int64_t incx, const double *y,
+                int64_t incy, double *a, int64_t lda, coral_stream_t stream);
+/* coral-aarch64/src/level2/cher.rs:62 / zher.rs:62: A += alpha * x * x^H, alpha real */
+int coral_cher(int32_t uplo, int64_t n, float alpha, const float *x, int64_t incx, float *a, int64_t lda,
+               coral_stream_t stream);
+int coral_zher(int32_t uplo, int64_t n, double alpha, const double *x, int64_t incx, double *a, int64_t lda,
+               coral_stream_t stream);
+/* coral-aarch64/src/level2/cher2.rs:65 / zher2.rs:63: A += alpha * x * y^H + conj(alpha) * y * x^H */
+int coral_cher2(int32_t uplo, int64_t n, const float alpha[2], const float *x, int64_t incx, const float *y,
+                int64_t incy, float *a, int64_t lda, coral_stream_t stream);
+int coral_zher2(int32_t uplo, int64_t n, const double alpha[2], const double *x, int64_t incx, const double *y,
+                int64_t incy, double *a, int64_t lda, coral_stream_t stream);
+
 /* ======================================================================== */
 /* Level 1 (results of reductions are written to DEVICE memory)             */
 /* ======================================================================== */
@@ -309,6 +349,30 @@ int coralh_strmv(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const flo
 int coralh_dtrmv(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const double *a, int64_t lda, double *x, int64_t incx);
 int coralh_ctrmv(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const float *a, int64_t lda, float *x, int64_t incx);
 int coralh_ztrmv(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const double *a, int64_t lda, double *x, int64_t incx);
+int coralh_sger(int64_t m, int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy,
+                float *a, int64_t lda);
+int coralh_dger(int64_t m, int64_t n, double alpha, const double *x, int64_t incx, const double *y, int64_t incy,
+                double *a, int64_t lda);
+int coralh_cgeru(int64_t m, int64_t n, const float alpha[2], const float *x, int64_t incx, const float *y,
+                 int64_t incy, float *a, int64_t lda);
+int coralh_cgerc(int64_t m, int64_t n, const float alpha[2], const float *x, int64_t incx, const float *y,
+                 int64_t incy, float *a, int64_t lda);
+int coralh_zgeru(int64_t m, int64_t n, const double alpha[2], const double *x, int64_t incx, const double *y,
+                 int64_t incy, double *a, int64_t lda);
+int coralh_zgerc(int64_t m, int64_t n, const double alpha[2], const double *x, int64_t incx, const double *y,
+                 int64_t incy, double *a, int64_t lda);
+int coralh_ssyr(int32_t uplo, int64_t n, float alpha, const float *x, int64_t incx, float *a, int64_t lda);
+int coralh_dsyr(int32_t uplo, int64_t n, double alpha, const double *x, int64_t incx, double *a, int64_t lda);
+int coralh_ssyr2(int32_t uplo, int64_t n, float alpha, const float *x, int64_t incx, const float *y, int64_t incy,
+                 float *a, int64_t lda);
+int coralh_dsyr2(int32_t uplo, int64_t n, double alpha, const double *x, int64_t incx, const double *y,
+                 int64_t incy, double *a, int64_t lda);
+int coralh_cher(int32_t uplo, int64_t n, float alpha, const float *x, int64_t incx, float *a, int64_t lda);
+int coralh_zher(int32_t uplo, int64_t n, double alpha, const double *x, int64_t incx, double *a, int64_t lda);
+int coralh_cher2(int32_t uplo, int64_t n, const float alpha[2], const float *x, int64_t incx, const float *y,
+                 int64_t incy, float *a, int64_t lda);
+int coralh_zher2(int32_t uplo, int64_t n, const double alpha[2], const double *x, int64_t incx, const double *y,
+                 int64_t incy, double *a, int64_t lda);
 int coralh_sdot(int64_t n, const float *x, int64_t incx, const float *y, int64_t incy, float *result);
 int coralh_ddot(int64_t n, const double *x, int64_t incx, const double *y, int64_t incy, double *result);
 int coralh_cdotu(int64_t n, const float *x, int64_t incx, const float *y, int64_t incy, float *result);

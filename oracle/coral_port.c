@@ -617,25 +617,203 @@ DEFINE_CDOT(zdotu, double, 0)
 DEFINE_CDOT(zdotc, double, 1)
 
 /* daxpy.rs:42-132: quick return n == 0 or alpha == 0 (:51-53); fused body. */
-#define DEFINE_AXPY(NAME, T, FMA)                                                                            \
+/* How CORAL's axpy family rounds one element update y += c*x:                */
+/*   AXPY_UNIT  unit-stride axpy: the NEON body (fused) covers the first      */
+/*              floor(len/Q)*Q elements, the scalar tail is `y += c*x` (two   */
+/*              roundings).  Q = 8 saxpy (coral-aarch64/src/level1/saxpy.rs:  */
+/*              63-112), 4 daxpy (daxpy.rs:63-112), 4 caxpy (caxpy.rs:70-146), */
+/*              2 zaxpy (zaxpy.rs:69-145).                                     */
+/*   AXPY_PLAIN strided axpy (saxpy.rs:114-126 and alike): two roundings.      */
+/*   AXPY_FUSED the inline `x.mul_add(c, a)` loops of the strided real rank    */
+/*              updates: fused everywhere.                                    */
+/* A fused complex update is yr = fma(ar,xr,yr); yr = fma(-ai,xi,yr);          */
+/* yi = fma(ar,xi,yi); yi = fma(ai,xr,yi) (vfmaq/vfmsq order); a plain one is  */
+/* yr += ar*xr - ai*xi; yi += ar*xi + ai*xr.                                   */
+#define AXPY_PLAIN 0
+#define AXPY_UNIT 1
+#define AXPY_FUSED 2
+
+#define DEFINE_AXPY_SEG(NAME, T, FMA, Q)                                                                     \
+  static void NAME##_seg(int64_t len, T c, const T *x, int64_t incx, T *y, int64_t incy, int mode) {        \
+    int64_t nf = mode == AXPY_FUSED ? len : mode == AXPY_UNIT ? len / Q * Q : 0;                             \
+    for (int64_t i = 0; i < len; ++i) {                                                                      \
+      T *yp = y + i * incy, xv = x[i * incx];                                                                \
+      *yp = i < nf ? FMA(xv, c, *yp) : *yp + c * xv;                                                         \
+    }                                                                                                        \
+  }
+DEFINE_AXPY_SEG(saxpy, float, fmaf, 8)
+DEFINE_AXPY_SEG(daxpy, double, fma, 4)
+
+#define DEFINE_CAXPY_SEG(NAME, T, FMA, Q)                                                                    \
+  static void NAME##_seg(int64_t len, T ar, T ai, const T *x, int64_t incx, T *y, int64_t incy, int mode) { \
+    int64_t nf = mode == AXPY_FUSED ? len : mode == AXPY_UNIT ? len / Q * Q : 0;                             \
+    for (int64_t i = 0; i < len; ++i) {                                                                      \
+      T xr = x[2 * i * incx], xi = x[2 * i * incx + 1];                                                      \
+      T yr = y[2 * i * incy], yi = y[2 * i * incy + 1];                                                      \
+      if (i < nf) {                                                                                          \
+        yr = FMA(ar, xr, yr);                                                                                \
+        yr = FMA(-ai, xi, yr);                                                                               \
+        yi = FMA(ar, xi, yi);                                                                                \
+        yi = FMA(ai, xr, yi);                                                                                \
+      } else {                                                                                               \
+        yr += ar * xr - ai * xi;                                                                             \
+        yi += ar * xi + ai * xr;                                                                             \
+      }                                                                                                      \
+      y[2 * i * incy] = yr;                                                                                  \
+      y[2 * i * incy + 1] = yi;                                                                              \
+    }                                                                                                        \
+  }
+DEFINE_CAXPY_SEG(caxpy, float, fmaf, 4)
+DEFINE_CAXPY_SEG(zaxpy, double, fma, 2)
+
+static int axpy_mode(int64_t incx, int64_t incy) { return incx == 1 && incy == 1 ? AXPY_UNIT : AXPY_PLAIN; }
+
+/* level1/saxpy.rs:42-131, daxpy.rs: quick return n == 0 || alpha == 0. */
+#define DEFINE_AXPY(NAME, T)                                                                                 \
   void port_##NAME(int64_t n, T alpha, const T *x, int64_t incx, T *y, int64_t incy) {                      \
     if (n == 0 || alpha == (T)0) return;                                                                     \
-    for (int64_t i = 0; i < n; ++i) y[i * incy] = FMA(alpha, x[i * incx], y[i * incy]);                      \
+    NAME##_seg(n, alpha, x, incx, y, incy, axpy_mode(incx, incy));                                           \
   }
-DEFINE_AXPY(saxpy, float, fmaf)
-DEFINE_AXPY(daxpy, double, fma)
+DEFINE_AXPY(saxpy, float)
+DEFINE_AXPY(daxpy, double)
 
+/* level1/caxpy.rs:45-172, zaxpy.rs: quick return n == 0 || alpha == 0. */
 #define DEFINE_CAXPY(NAME, T)                                                                                \
   void port_##NAME(int64_t n, const T *alpha, const T *x, int64_t incx, T *y, int64_t incy) {               \
     if (n == 0 || (alpha[0] == 0 && alpha[1] == 0)) return;                                                  \
-    for (int64_t i = 0; i < n; ++i) {                                                                        \
-      T xr = x[2 * i * incx], xi = x[2 * i * incx + 1];                                                      \
-      y[2 * i * incy] += alpha[0] * xr - alpha[1] * xi;                                                      \
-      y[2 * i * incy + 1] += alpha[0] * xi + alpha[1] * xr;                                                  \
-    }                                                                                                        \
+    NAME##_seg(n, alpha[0], alpha[1], x, incx, y, incy, axpy_mode(incx, incy));                              \
   }
 DEFINE_CAXPY(caxpy, float)
 DEFINE_CAXPY(zaxpy, double)
+
+/* ======================================================================== */
+/* Level-2 rank updates: per column a coefficient, skipped when zero, then  */
+/* an axpy into the stored rows of that column.  uplo 0 = upper (rows 0..j), */
+/* 1 = lower (rows j..n-1).                                                  */
+/* ======================================================================== */
+static int64_t seg_start(int32_t uplo, int64_t j) { return uplo == 0 ? 0 : j; }
+static int64_t seg_len(int32_t uplo, int64_t n, int64_t j) { return uplo == 0 ? j + 1 : n - j; }
+
+/* coral-aarch64/src/level2/sger.rs:68-132, dger.rs:69-133: A += x * (alpha*y_j).  Quick return m == 0 ||    */
+/* n == 0 || alpha == 0; a column with alpha*y_j == 0 is skipped.  Unit      */
+/* strides go through saxpy (AXPY_UNIT), others through mul_add (FUSED).     */
+#define DEFINE_GER(NAME, AXPY, T)                                                                            \
+  void port_##NAME(int64_t m, int64_t n, T alpha, const T *x, int64_t incx, const T *y, int64_t incy, T *a, \
+                   int64_t lda) {                                                                            \
+    if (m == 0 || n == 0 || alpha == (T)0) return;                                                           \
+    int mode = incx == 1 && incy == 1 ? AXPY_UNIT : AXPY_FUSED;                                              \
+    for (int64_t j = 0; j < n; ++j) {                                                                        \
+      T c = alpha * y[j * incy];                                                                             \
+      if (c != (T)0) AXPY##_seg(m, c, x, incx, a + j * lda, 1, mode);                                        \
+    }                                                                                                        \
+  }
+DEFINE_GER(sger, saxpy, float)
+DEFINE_GER(dger, daxpy, double)
+
+/* level2/cgeru.rs:63-144, cgerc.rs:62-143, zgeru.rs:61-142, zgerc.rs:60-141:
+   coefficient alpha*y_j (u) or         */
+/* alpha*conj(y_j) (c), skipped when zero.  Unit strides go through caxpy    */
+/* (AXPY_UNIT); others use an inline non-fused loop (AXPY_PLAIN).           */
+#define DEFINE_CGER(NAME, AXPY, T, CONJ)                                                                     \
+  void port_##NAME(int64_t m, int64_t n, const T *alpha, const T *x, int64_t incx, const T *y, int64_t incy,\
+                   T *a, int64_t lda) {                                                                      \
+    if (m == 0 || n == 0 || (alpha[0] == 0 && alpha[1] == 0)) return;                                        \
+    int mode = axpy_mode(incx, incy);                                                                        \
+    for (int64_t j = 0; j < n; ++j) {                                                                        \
+      T yr = y[2 * j * incy], yi = y[2 * j * incy + 1];                                                      \
+      T cr = CONJ ? alpha[0] * yr + alpha[1] * yi : alpha[0] * yr - alpha[1] * yi;                           \
+      T ci = CONJ ? -alpha[0] * yi + alpha[1] * yr : alpha[0] * yi + alpha[1] * yr;                          \
+      if (cr != 0 || ci != 0) AXPY##_seg(m, cr, ci, x, incx, a + 2 * j * lda, 1, mode);                      \
+    }                                                                                                        \
+  }
+DEFINE_CGER(cgeru, caxpy, float, 0)
+DEFINE_CGER(cgerc, caxpy, float, 1)
+DEFINE_CGER(zgeru, zaxpy, double, 0)
+DEFINE_CGER(zgerc, zaxpy, double, 1)
+
+/* level2/ssyr.rs:63-181, dsyr.rs:61-179: aj = alpha*x_j (skipped when zero), then the     */
+/* stored rows of column j += x*aj; incx == 1 through saxpy, else mul_add.   */
+#define DEFINE_SYR(NAME, AXPY, T)                                                                            \
+  void port_##NAME(int32_t uplo, int64_t n, T alpha, const T *x, int64_t incx, T *a, int64_t lda) {         \
+    if (n == 0 || alpha == (T)0) return;                                                                     \
+    int mode = incx == 1 ? AXPY_UNIT : AXPY_FUSED;                                                           \
+    for (int64_t j = 0; j < n; ++j) {                                                                        \
+      T aj = alpha * x[j * incx];                                                                            \
+      int64_t i0 = seg_start(uplo, j);                                                                       \
+      if (aj != (T)0) AXPY##_seg(seg_len(uplo, n, j), aj, x + i0 * incx, incx, a + i0 + j * lda, 1, mode);   \
+    }                                                                                                        \
+  }
+DEFINE_SYR(ssyr, saxpy, float)
+DEFINE_SYR(dsyr, daxpy, double)
+
+/* level2/ssyr2.rs:59-197, dsyr2.rs:63-201: aj_y = alpha*y_j, aj_x = alpha*x_j.  Unit     */
+/* strides: saxpy of x with aj_y, then of y with aj_x, each skipped when     */
+/* zero.  Otherwise, unless both are zero, one mul_add loop applies both     */
+/* (a = fma(x_i, aj_y, a); a = fma(y_i, aj_x, a)), zero coefficients too.    */
+#define DEFINE_SYR2(NAME, AXPY, T)                                                                           \
+  void port_##NAME(int32_t uplo, int64_t n, T alpha, const T *x, int64_t incx, const T *y, int64_t incy,    \
+                   T *a, int64_t lda) {                                                                      \
+    if (n == 0 || alpha == (T)0) return;                                                                     \
+    int unit = incx == 1 && incy == 1;                                                                       \
+    for (int64_t j = 0; j < n; ++j) {                                                                        \
+      T ajy = alpha * y[j * incy], ajx = alpha * x[j * incx];                                                \
+      int64_t i0 = seg_start(uplo, j), len = seg_len(uplo, n, j);                                            \
+      T *col = a + i0 + j * lda;                                                                             \
+      if (unit) {                                                                                            \
+        if (ajy != (T)0) AXPY##_seg(len, ajy, x + i0, 1, col, 1, AXPY_UNIT);                                 \
+        if (ajx != (T)0) AXPY##_seg(len, ajx, y + i0, 1, col, 1, AXPY_UNIT);                                 \
+      } else if (ajy != (T)0 || ajx != (T)0) {                                                               \
+        AXPY##_seg(len, ajy, x + i0 * incx, incx, col, 1, AXPY_FUSED);                                       \
+        AXPY##_seg(len, ajx, y + i0 * incy, incy, col, 1, AXPY_FUSED);                                       \
+      }                                                                                                      \
+    }                                                                                                        \
+  }
+DEFINE_SYR2(ssyr2, saxpy, float)
+DEFINE_SYR2(dsyr2, daxpy, double)
+
+/* level2/cher.rs:62-212, zher.rs:62-212 (alpha real): aj = [alpha*xr_j, -alpha*xi_j],     */
+/* skipped when zero; caxpy(x, incx) into the stored rows of column j (fused */
+/* body only for incx == 1), then Im(a_jj) = 0 for the updated column.       */
+#define DEFINE_HER(NAME, AXPY, T)                                                                            \
+  void port_##NAME(int32_t uplo, int64_t n, T alpha, const T *x, int64_t incx, T *a, int64_t lda) {         \
+    if (n == 0 || alpha == (T)0) return;                                                                     \
+    for (int64_t j = 0; j < n; ++j) {                                                                        \
+      T ar = alpha * x[2 * j * incx], ai = -alpha * x[2 * j * incx + 1];                                     \
+      if (ar == 0 && ai == 0) continue;                                                                      \
+      int64_t i0 = seg_start(uplo, j);                                                                       \
+      AXPY##_seg(seg_len(uplo, n, j), ar, ai, x + 2 * i0 * incx, incx, a + 2 * (i0 + j * lda), 1,            \
+                 axpy_mode(incx, 1));                                                                        \
+      a[2 * (j + j * lda) + 1] = 0;                                                                          \
+    }                                                                                                        \
+  }
+DEFINE_HER(cher, caxpy, float)
+DEFINE_HER(zher, zaxpy, double)
+
+/* level2/cher2.rs:65-311, zher2.rs:63-309: aj_y = alpha*conj(y_j), aj_x =                */
+/* conj(alpha)*conj(x_j); caxpy(x, incx) with aj_y, then caxpy(y, incy) with */
+/* aj_x, each skipped when zero (fused body only for a unit source stride).  */
+/* Im(a_jj) = 0: always for unit strides, otherwise only when a coefficient  */
+/* is nonzero.                                                               */
+#define DEFINE_HER2(NAME, AXPY, T)                                                                           \
+  void port_##NAME(int32_t uplo, int64_t n, const T *alpha, const T *x, int64_t incx, const T *y,          \
+                   int64_t incy, T *a, int64_t lda) {                                                        \
+    if (n == 0 || (alpha[0] == 0 && alpha[1] == 0)) return;                                                  \
+    T ar = alpha[0], ai = alpha[1];                                                                          \
+    int unit = incx == 1 && incy == 1;                                                                       \
+    for (int64_t j = 0; j < n; ++j) {                                                                        \
+      T xr = x[2 * j * incx], xi = x[2 * j * incx + 1], yr = y[2 * j * incy], yi = y[2 * j * incy + 1];      \
+      T ayr = ar * yr + ai * yi, ayi = -ar * yi + ai * yr;                                                   \
+      T axr = ar * xr - ai * xi, axi = -ar * xi - ai * xr;                                                   \
+      int any_y = ayr != 0 || ayi != 0, any_x = axr != 0 || axi != 0;                                        \
+      int64_t i0 = seg_start(uplo, j), len = seg_len(uplo, n, j);                                            \
+      T *col = a + 2 * (i0 + j * lda);                                                                       \
+      if (any_y) AXPY##_seg(len, ayr, ayi, x + 2 * i0 * incx, incx, col, 1, axpy_mode(incx, 1));             \
+      if (any_x) AXPY##_seg(len, axr, axi, y + 2 * i0 * incy, incy, col, 1, axpy_mode(incy, 1));             \
+      if (unit || any_y || any_x) a[2 * (j + j * lda) + 1] = 0;                                              \
+    }                                                                                                        \
+  }
+DEFINE_HER2(cher2, caxpy, float)
+DEFINE_HER2(zher2, zaxpy, double)
 
 /* snrm2.rs:45-149 with upd_f32 (nrm2_helpers.rs:15-38): scale = 0, ssq = 1;  */
 /* chunks of 16 (f32) / 8 (f64): chunk max, normalise by 1/max, fused ssq,  */

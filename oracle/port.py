@@ -50,6 +50,19 @@ for _p in "sdcz":
     _SIGS[f"port_i{_p}amax"] = (i64, [i64, P, i64])
 for _n in ("cdotu", "cdotc", "zdotu", "zdotc"):
     _SIGS[f"port_{_n}"] = (None, [i64, P, i64, P, i64, P])
+# level-2 rank updates
+_SIGS["port_sger"] = (None, [i64, i64, f32, P, i64, P, i64, P, i64])
+_SIGS["port_dger"] = (None, [i64, i64, f64, P, i64, P, i64, P, i64])
+for _n in ("cgeru", "cgerc", "zgeru", "zgerc"):
+    _SIGS[f"port_{_n}"] = (None, [i64, i64, P, P, i64, P, i64, P, i64])
+_SIGS["port_ssyr"] = (None, [i32, i64, f32, P, i64, P, i64])
+_SIGS["port_dsyr"] = (None, [i32, i64, f64, P, i64, P, i64])
+_SIGS["port_cher"] = (None, [i32, i64, f32, P, i64, P, i64])
+_SIGS["port_zher"] = (None, [i32, i64, f64, P, i64, P, i64])
+_SIGS["port_ssyr2"] = (None, [i32, i64, f32, P, i64, P, i64, P, i64])
+_SIGS["port_dsyr2"] = (None, [i32, i64, f64, P, i64, P, i64, P, i64])
+_SIGS["port_cher2"] = (None, [i32, i64, P, P, i64, P, i64, P, i64])
+_SIGS["port_zher2"] = (None, [i32, i64, P, P, i64, P, i64, P, i64])
 
 _lib = None
 
@@ -171,6 +184,38 @@ def _mk_axpy(p):
 
 
 saxpy, daxpy, caxpy, zaxpy = (_mk_axpy(p) for p in "sdcz")
+
+
+def _mk_ger(p, name):
+    def f(m, n, alpha, x, incx, y, incy, a, lda):
+        al = _s2(alpha, p).ctypes.data if p in "cz" else alpha
+        getattr(load(), f"port_{name}")(m, n, al, _ptr(x), incx, _ptr(y), incy, _ptr(a), lda)
+    return f
+
+
+sger, dger = _mk_ger("s", "sger"), _mk_ger("d", "dger")
+cgeru, cgerc = _mk_ger("c", "cgeru"), _mk_ger("c", "cgerc")
+zgeru, zgerc = _mk_ger("z", "zgeru"), _mk_ger("z", "zgerc")
+
+
+def _mk_syr(name):
+    def f(uplo, n, alpha, x, incx, a, lda):
+        getattr(load(), f"port_{name}")(int(uplo), n, alpha, _ptr(x), incx, _ptr(a), lda)
+    return f
+
+
+ssyr, dsyr, cher, zher = (_mk_syr(nm) for nm in ("ssyr", "dsyr", "cher", "zher"))
+
+
+def _mk_syr2(p, name):
+    def f(uplo, n, alpha, x, incx, y, incy, a, lda):
+        al = _s2(alpha, p).ctypes.data if p in "cz" else alpha
+        getattr(load(), f"port_{name}")(int(uplo), n, al, _ptr(x), incx, _ptr(y), incy, _ptr(a), lda)
+    return f
+
+
+ssyr2, dsyr2 = _mk_syr2("s", "ssyr2"), _mk_syr2("d", "dsyr2")
+cher2, zher2 = _mk_syr2("c", "cher2"), _mk_syr2("z", "zher2")
 
 
 def snrm2(n, x, incx):

@@ -23,6 +23,9 @@ def test_header_declares_the_call_surface():
               "cgemm_3m_fp16x3", "sgemm_batched",
               "dgemm_batched", "dtrsv_batched", "autotune_run", "autotune_save", "autotune_load"):
         assert f"coral_{r}" in names, r
+    for r in ("sger", "dger", "cgeru", "cgerc", "zgeru", "zgerc", "ssyr", "dsyr", "cher", "zher",
+              "ssyr2", "dsyr2", "cher2", "zher2"):
+        assert f"coral_{r}" in names and f"coralh_{r}" in names, r
 
 
 def test_library_exports_every_declared_symbol():
@@ -67,6 +70,37 @@ def test_level2_level1_validation_codes():
     assert _status("coral_ddot", -1, p, 1, p, 1, p, None) == -1
     assert _status("coral_ddot", 4, p, 0, p, 1, p, None) == -3
     assert _status("coral_dnrm2", 4, p, 1, None, None) == -4
+
+
+def test_rank_update_validation_codes():
+    buf = np.zeros(64)
+    p = buf.ctypes.data
+    alv, zv = np.array([1.0, 0.5]), np.zeros(2)  # kept alive while their pointers are used
+    al, zero = alv.ctypes.data, zv.ctypes.data
+    # ger: m=1 n=2 alpha=3 x=4 incx=5 y=6 incy=7 a=8 lda=9
+    assert _status("coral_dger", -1, 4, 1.0, p, 1, p, 1, p, 4, None) == -1
+    assert _status("coral_dger", 4, -1, 1.0, p, 1, p, 1, p, 4, None) == -2
+    assert _status("coral_dger", 4, 4, 1.0, p, 0, p, 1, p, 4, None) == -5
+    assert _status("coral_dger", 4, 4, 1.0, p, 1, p, 0, p, 4, None) == -7
+    assert _status("coral_dger", 4, 4, 1.0, p, 1, p, 1, p, 3, None) == -9
+    assert _status("coral_dger", 4, 4, 1.0, None, 1, p, 1, p, 4, None) == -4
+    assert _status("coral_zgerc", 4, 4, None, p, 1, p, 1, p, 4, None) == -3
+    assert _status("coral_zgeru", 4, 4, al, p, 1, None, 1, p, 4, None) == -6
+    # syr / her: uplo=1 n=2 alpha=3 x=4 incx=5 a=6 lda=7
+    assert _status("coral_dsyr", 2, 4, 1.0, p, 1, p, 4, None) == -1
+    assert _status("coral_dsyr", 0, -1, 1.0, p, 1, p, 4, None) == -2
+    assert _status("coral_zher", 0, 4, 1.0, p, 0, p, 4, None) == -5
+    assert _status("coral_zher", 1, 4, 1.0, p, 1, p, 3, None) == -7
+    assert _status("coral_dsyr", 1, 4, 1.0, p, 1, None, 4, None) == -6
+    # syr2 / her2: uplo=1 n=2 alpha=3 x=4 incx=5 y=6 incy=7 a=8 lda=9
+    assert _status("coral_dsyr2", 0, 4, 1.0, p, 1, p, 0, p, 4, None) == -7
+    assert _status("coral_zher2", 0, 4, None, p, 1, p, 1, p, 4, None) == -3
+    assert _status("coral_zher2", 0, 4, al, p, 1, p, 1, p, 2, None) == -9
+    # quick returns (sger.rs:80, ssyr.rs:73, cher2.rs:77) need no device
+    assert _status("coral_dger", 0, 4, 1.0, p, 1, p, 1, p, 1, None) == 0
+    assert _status("coral_dger", 4, 4, 0.0, p, 1, p, 1, p, 4, None) == 0
+    assert _status("coral_dsyr", 0, 0, 1.0, p, 1, p, 1, None) == 0
+    assert _status("coral_zher2", 1, 4, zero, p, 1, p, 1, p, 4, None) == 0
 
 
 def test_quick_returns_need_no_device():

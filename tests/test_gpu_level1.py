@@ -49,26 +49,28 @@ def test_dot(gpu_lib):
             assert abs(complex(*rr) - e) <= 8 * 2.0 ** -24 * r * sc + 1e-30, name
 
 
-def test_axpy(gpu_lib):
+@pytest.mark.parametrize("p", "sdcz")
+def test_axpy(gpu_lib, p):
+    """Bit-identical to the oracle: fused over the NEON body of a unit-stride call,
+    twice rounded in its tail and in strided calls (saxpy.rs:63-126)."""
     rng = np.random.default_rng(31)
-    for n, incx, incy in ((0, 1, 1), (1, 1, 1), (1000, 1, 1), (777, 3, 2)):
-        x = rng.uniform(-1, 1, (n - 1) * incx + 1 if n else 1)
-        y = rng.uniform(-1, 1, (n - 1) * incy + 1 if n else 1)
+    dt = {"s": np.float32, "d": np.float64, "c": np.complex64, "z": np.complex128}[p]
+    cplx = p in "cz"
+    alpha = (0.5, -2.0) if cplx else 0.5
+    for n, incx, incy, off in ((0, 1, 1, 0), (1, 1, 1, 0), (13, 1, 1, 0), (1000, 1, 1, 0), (1003, 1, 1, 0),
+                               (1003, 1, 1, 1), (777, 3, 2, 0), (777, 1, 2, 0)):
+        lx, ly = (n - 1) * incx + 1 + off if n else 1, (n - 1) * incy + 1 + off if n else 1
+        x = rng.uniform(-1, 1, lx) + (1j * rng.uniform(-1, 1, lx) if cplx else 0)
+        y = rng.uniform(-1, 1, ly) + (1j * rng.uniform(-1, 1, ly) if cplx else 0)
+        x, y = x.astype(dt), y.astype(dt)
         out = y.copy()
-        blas.daxpy(n, 0.5, x, incx, out, incy)
+        getattr(blas, f"{p}axpy")(n, alpha, x[off:], incx, out[off:], incy)  # off=1: unaligned, unit stride
         po = y.copy()
-        port.daxpy(n, 0.5, x, incx, po, incy)
-        assert np.array_equal(out, po)  # both one fma per element
-        out = y.copy()
-        blas.daxpy(n, 0.0, np.full_like(x, np.nan), incx, out, incy)  # alpha == 0 quick return
-        assert np.array_equal(out, y)
-        xc = (x + 0.5j * x).astype(np.complex64)
-        yc = (y - 1j * y).astype(np.complex64)
-        out = yc.copy()
-        blas.caxpy(n, (0.5, -2.0), xc, incx, out, incy)
-        ref = yc.astype(np.complex128).copy()
-        ref[: (n - 1) * incy + 1: incy] += (0.5 - 2.0j) * xc[: (n - 1) * incx + 1: incx] if n else 0
-        assert np.abs(out - ref).max() <= 1e-6 if n else True
+        getattr(port, f"{p}axpy")(n, alpha, x[off:], incx, po[off:], incy)
+        assert np.array_equal(out.view(np.uint8), po.view(np.uint8)), (p, n, incx, incy, off)
+    out = y.copy()
+    getattr(blas, f"{p}axpy")(n, (0.0, 0.0) if cplx else 0.0, np.full_like(x, np.nan), incx, out, incy)
+    assert np.array_equal(out, y)  # alpha == 0 quick return
 
 
 def test_nrm2(gpu_lib):
