@@ -517,7 +517,7 @@ def suite_c5():
            "gbps": round(by / ms / 1e6, 1), "frac_hbm": round(by / ms / 1e6 / PEAK_HBM, 4), "cpu_baseline": cpu})
 
 
-def suite_rows():
+def suite_rows(parts=("gemv", "symv", "tr", "rank", "l1")):
     """The remaining routines of the call surface, one line each, at sizes that
     stream well past the caches: GB/s of algorithmic bytes against HBM, with the
     CORAL algorithm on one core beside each on a bounded sample."""
@@ -551,7 +551,7 @@ def suite_rows():
         return v.astype(dt)
 
     # ---- GEMV (a8, a9): N and T/C ----
-    for dt, p, m in ((np.float32, "s", 16384), (np.complex64, "c", 8192), (np.complex128, "z", 8192)):
+    for dt, p, m in () if "gemv" not in parts else ((np.float32, "s", 16384), (np.complex64, "c", 8192), (np.complex128, "z", 8192)):
         es = np.dtype(dt).itemsize
         A, x, y = device.DeviceArray(m * m, dt), device.DeviceArray(m, dt), device.DeviceArray(m, dt)
         device.fill_random(A, 71, "uniform")
@@ -566,7 +566,7 @@ def suite_rows():
             emit(f"{p.upper()}GEMV {'NTC'[tr]} {m}^2", ms, es * (m * m + 2 * m), cpu)
         del A, x, y
     # ---- SYMV / HEMV (a10, a11): stored triangle ----
-    for dt, name, n, uplo in ((np.float32, "ssymv", 16384, 0), (np.float64, "dsymv", 16384, 0),
+    for dt, name, n, uplo in () if "symv" not in parts else ((np.float32, "ssymv", 16384, 0), (np.float64, "dsymv", 16384, 0),
                               (np.complex128, "zhemv", 8192, 1)):
         es = np.dtype(dt).itemsize
         A, x, y = device.DeviceArray(n * n, dt), device.DeviceArray(n, dt), device.DeviceArray(n, dt)
@@ -583,8 +583,9 @@ def suite_rows():
     # ---- TRSV / TRMV (a12-a14): strictly-lower U[-1,1]/n, diagonal U[1,2] ----
     n = 8192
     rng = np.random.default_rng(79)
-    base = np.tril(rng.uniform(-1, 1, (n, n)) / n, -1) + np.diag(rng.uniform(1, 2, n))
-    for dt, p in ((np.float32, "s"), (np.float64, "d"), (np.complex64, "c"), (np.complex128, "z")):
+    base = np.tril(rng.uniform(-1, 1, (n, n)) / n, -1) + np.diag(rng.uniform(1, 2, n)) if "tr" in parts else None
+    for dt, p in () if "tr" not in parts else ((np.float32, "s"), (np.float64, "d"), (np.complex64, "c"),
+                                               (np.complex128, "z")):
         es = np.dtype(dt).itemsize
         Ah = base.astype(dt)
         if p in "cz":
@@ -622,10 +623,42 @@ def suite_rows():
                                   nbytes, "GB/s", 1e9, f"D{routine.upper()} lower {'NT'[trans]} n={n}")
                 emit(f"{p.upper()}{routine.upper()} lower {'NT'[trans]} non-unit {n}", ms, nbytes, cpu, extra)
         del A, X, Ah, af
+    # ---- rank updates: A read and written once (stored part) ----
+    for dt, name, n in () if "rank" not in parts else ((np.float32, "sger", 16384), (np.float64, "dger", 16384), (np.complex64, "cgeru", 8192),
+                        (np.complex128, "zgerc", 8192), (np.float64, "dsyr", 16384), (np.float64, "dsyr2", 16384),
+                        (np.complex64, "cher", 8192), (np.complex128, "zher2", 8192)):
+        es = np.dtype(dt).itemsize
+        cplx = dt in cplx_arg
+        A, x, y = device.DeviceArray(n * n, dt), device.DeviceArray(n, dt), device.DeviceArray(n, dt)
+        device.fill_random(A, 85, "uniform")
+        device.fill_random(x, 86, "uniform")
+        device.fill_random(y, 87, "uniform")
+        hs = 4096
+        ha, hx, hy = host_rand(dt, hs * hs, 88), host_rand(dt, hs, 89), host_rand(dt, hs, 90)
+        al = 1e-3 if (name.endswith("syr") or name.endswith("her")) else scal(dt, 1e-3)
+        hal = 1e-3 if (name.endswith("syr") or name.endswith("her") or not cplx) else (1e-3, 0.0)
+        if name.endswith("ger") or name[1:4] == "ger":
+            fn = lambda: call(f"coral_{name}", n, n, al, x.ptr, 1, y.ptr, 1, A.ptr, n, None)
+            hfn = lambda: getattr(port, name)(hs, hs, hal, hx, 1, hy, 1, ha, hs)
+            stored, label = lambda k: k * k, f"{name.upper()} {n}^2"
+        elif name.endswith("2"):
+            fn = lambda: call(f"coral_{name}", 0, n, al, x.ptr, 1, y.ptr, 1, A.ptr, n, None)
+            hfn = lambda: getattr(port, name)(0, hs, hal, hx, 1, hy, 1, ha, hs)
+            stored, label = lambda k: k * (k + 1) / 2, f"{name.upper()} upper {n}"
+        else:
+            fn = lambda: call(f"coral_{name}", 1, n, al, x.ptr, 1, A.ptr, n, None)
+            hfn = lambda: getattr(port, name)(1, hs, hal, hx, 1, ha, hs)
+            stored, label = lambda k: k * (k + 1) / 2, f"{name.upper()} lower {n}"
+        nvec = 1 if name.endswith("syr") or name.endswith("her") else 2
+        nbytes = lambda k: es * (2 * stored(k) + nvec * k)
+        ms = timed_steps(fn, 20, 3)
+        cpu = cpu_leg(hfn, nbytes(hs), "GB/s", 1e9, f"{name.upper()} n={hs}")
+        emit(label, ms, nbytes(n), cpu)
+        del A, x, y
     # ---- level 1 (a15-a19), n = 2^27 ----
     n = 1 << 27
     hn = 1 << 22
-    for dt, p in ((np.float32, "s"), (np.float64, "d"), (np.complex128, "z")):
+    for dt, p in () if "l1" not in parts else ((np.float32, "s"), (np.float64, "d"), (np.complex128, "z")):
         es = np.dtype(dt).itemsize
         x, y = device.DeviceArray(n, dt), device.DeviceArray(n, dt)
         device.fill_random(x, 81, "uniform")
@@ -660,7 +693,8 @@ def run_suite(which):
     lib = load()
     if lib.coral_init() != 0:
         raise SystemExit(f"coral_init failed: {lib.coral_last_error().decode()}")
-    table = {"c1": suite_c1, "c2": suite_c2, "c3": suite_c3, "c4": suite_c4, "c5": suite_c5, "rows": suite_rows}
+    table = {"c1": suite_c1, "c2": suite_c2, "c3": suite_c3, "c4": suite_c4, "c5": suite_c5, "rows": suite_rows,
+             "rank": lambda: suite_rows(("rank",))}
     for w in which:
         log(f"suite {w} ...")
         table[w]()
@@ -675,7 +709,7 @@ def main():
     ap.add_argument("--gemv-n", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--suite", default="",
-                    help="comma list of c1,c2,c3,c4,c5,rows: one JSON line per configuration")
+                    help="comma list of c1,c2,c3,c4,c5,rows,rank: one JSON line per configuration")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
