@@ -13,8 +13,9 @@ behaviour in release builds on a bad length/stride).
 Reference signatures: level3/sgemm.rs:106, dgemm.rs:107, cgemm.rs:127, zgemm.rs:123;
 level2/dgemv.rs:61, cgemv.rs:67, dsymv.rs:99, chemv.rs:96, dtrsv.rs:62, dtrmv.rs:58,
 sger.rs:68, cgeru.rs:63, cgerc.rs:62, dsyr.rs:61, dsyr2.rs:63, cher.rs:62, cher2.rs:65;
-level1/sdot.rs:42, cdotc.rs:46, daxpy.rs:42, snrm2.rs:45, isamax.rs:43 (all under
-coral-aarch64/src/).
+level1/sdot.rs:42, cdotc.rs:46, daxpy.rs:42, snrm2.rs:45, isamax.rs:43, sscal.rs:30,
+csscal.rs:39, sasum.rs:40, scopy.rs:28, sswap.rs:35, srot.rs:51, srotm.rs:47, srotg.rs:40,
+srotmg.rs:42 (all under coral-aarch64/src/).
 """
 from __future__ import annotations
 
@@ -252,3 +253,90 @@ isamax = _iamax("s", "isamax")
 idamax = _iamax("d", "idamax")
 icamax = _iamax("c", "icamax")
 izamax = _iamax("z", "izamax")
+
+
+def _scal(p, name, real_alpha):
+    cplx_alpha = p in "cz" and not real_alpha
+
+    def f(n, alpha, x, incx):
+        al = _scalar2(alpha, _REAL[p]) if cplx_alpha else alpha
+        call(f"coralh_{name}", n, al, _buf(x, p, True), incx)
+
+    f.__name__ = name
+    return f
+
+
+sscal, dscal = _scal("s", "sscal", True), _scal("d", "dscal", True)
+cscal, zscal = _scal("c", "cscal", False), _scal("z", "zscal", False)
+csscal, zdscal = _scal("c", "csscal", True), _scal("z", "zdscal", True)
+
+sasum = _nrm2("s", "sasum")
+dasum = _nrm2("d", "dasum")
+scasum = _nrm2("c", "scasum")
+dzasum = _nrm2("z", "dzasum")
+
+
+def _xy(p, name, x_out):
+    def f(n, x, incx, y, incy):
+        call(f"coralh_{name}", n, _buf(x, p, x_out), incx, _buf(y, p, True), incy)
+
+    f.__name__ = name
+    return f
+
+
+scopy, dcopy, ccopy, zcopy = (_xy(p, f"{p}copy", False) for p in "sdcz")
+sswap, dswap, cswap, zswap = (_xy(p, f"{p}swap", True) for p in "sdcz")
+
+
+def _rot(p, name):
+    def f(n, x, incx, y, incy, c, s):
+        call(f"coralh_{name}", n, _buf(x, p, True), incx, _buf(y, p, True), incy, c, s)
+
+    f.__name__ = name
+    return f
+
+
+srot, drot, csrot, zdrot = _rot("s", "srot"), _rot("d", "drot"), _rot("c", "csrot"), _rot("z", "zdrot")
+
+
+def _rotm(p, name):
+    def f(n, x, incx, y, incy, param):
+        pa = np.ascontiguousarray(param, _REAL[p])
+        call(f"coralh_{name}", n, _buf(x, p, True), incx, _buf(y, p, True), incy, pa.ctypes.data)
+
+    f.__name__ = name
+    return f
+
+
+srotm, drotm = _rotm("s", "srotm"), _rotm("d", "drotm")
+
+
+def _rotg(p, name):
+    def f(a, b):
+        """CORAL's rotg on scalars: returns (r, z, c, s), the values CORAL leaves in a, b, c, s."""
+        v = np.array([a, b, 0, 0], _REAL[p])
+        base, sz = v.ctypes.data, v.itemsize
+        call(f"coralh_{name}", base, base + sz, base + 2 * sz, base + 3 * sz)
+        return tuple(v)
+
+    f.__name__ = name
+    return f
+
+
+srotg, drotg = _rotg("s", "srotg"), _rotg("d", "drotg")
+
+
+def _rotmg(p, name):
+    def f(d1, d2, x1, y1):
+        """CORAL's rotmg: returns (d1, d2, x1, param) as CORAL leaves them."""
+        v = np.array([d1, d2, x1], _REAL[p])
+        param = np.zeros(5, _REAL[p])
+        base, sz = v.ctypes.data, v.itemsize
+        call(f"coralh_{name}", base, base + sz, base + 2 * sz, y1, param.ctypes.data)
+        return v[0], v[1], v[2], param
+
+    f.__name__ = name
+    return f
+
+
+srotmg, drotmg = _rotmg("s", "srotmg"), _rotmg("d", "drotmg")

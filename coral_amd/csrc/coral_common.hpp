@@ -152,6 +152,10 @@ template <typename E> __device__ __forceinline__ E eshfl_xor(E v, int off) {
 // The body covers the first floor(len/Q)*Q elements of a unit-stride call
 // (coral-aarch64/src/level1/saxpy.rs:63-112 Q = 8, daxpy.rs Q = 4, caxpy.rs:70-132 Q = 4,
 // zaxpy.rs:69-131 Q = 2).
+// fused multiply-add in the element's own precision (no promotion of float to double)
+__device__ __forceinline__ float efma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double efma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename E> struct axpy_body;
 template <> struct axpy_body<float>   { static constexpr int64_t Q = 8; };
 template <> struct axpy_body<double>  { static constexpr int64_t Q = 4; };
@@ -174,10 +178,10 @@ template <typename E>
 __device__ __forceinline__ E axpy_upd_c(E y, E c, E x, bool fused) {
 #pragma clang fp contract(off)
   using R = typename elem_traits<E>::real;
-  R fr = __builtin_fma(c.x, x.x, y.x);
-  fr = __builtin_fma(-c.y, x.y, fr);
-  R fi = __builtin_fma(c.x, x.y, y.y);
-  fi = __builtin_fma(c.y, x.x, fi);
+  R fr = efma(c.x, x.x, y.x);
+  fr = efma(-c.y, x.y, fr);
+  R fi = efma(c.x, x.y, y.y);
+  fi = efma(c.y, x.x, fi);
   const R pr = y.x + (c.x * x.x - c.y * x.y);
   const R pi = y.y + (c.x * x.y + c.y * x.x);
   E r;

@@ -134,6 +134,57 @@ static int v2_host(size_t es, int64_t n, const void *x, int64_t incx, void *y, i
   return finish();
 }
 
+// level-1 with two vectors both written (swap / rot / rotm)
+template <typename F>
+static int xy_host(size_t es, int64_t n, void *x, int64_t incx, void *y, int64_t incy, F call) {
+  HOST_PROLOGUE();
+  if (n < 0 || incx <= 0 || incy <= 0) return call(x, y, st);
+  void *dx, *dy;
+  const size_t xb = vec_bytes(n, incx, es), yb = vec_bytes(n, incy, es);
+  CORAL_TRY(put(0, x, xb, &dx));
+  CORAL_TRY(put(1, y, yb, &dy));
+  int s = call(dx ? dx : x, dy ? dy : y, st);
+  if (s != CORAL_OK) { finish(); return s; }
+  CORAL_TRY(get(x, dx, xb));
+  CORAL_TRY(get(y, dy, yb));
+  return finish();
+}
+
+// level-1 with one vector in/out (scal)
+template <typename F>
+static int x_host(size_t es, int64_t n, void *x, int64_t incx, F call) {
+  HOST_PROLOGUE();
+  if (n < 0 || incx <= 0) return call(x, st);
+  void *dx;
+  const size_t xb = vec_bytes(n, incx, es);
+  CORAL_TRY(put(0, x, xb, &dx));
+  int s = call(dx ? dx : x, st);
+  if (s != CORAL_OK) { finish(); return s; }
+  CORAL_TRY(get(x, dx, xb));
+  return finish();
+}
+
+// scalar in/out arguments staged in one device buffer (rotg / rotmg)
+template <typename F>
+static int scalars_host(void *const *h, const size_t *bytes, int count, F call) {
+  HOST_PROLOGUE();
+  for (int i = 0; i < count; ++i)
+    if (!h[i]) return call((void *const *)h, st);  // null argument: reported without a device copy
+  void *d;
+  CORAL_TRY(buf(4, 64, &d));
+  void *dp[5];
+  size_t off = 0;
+  for (int i = 0; i < count; ++i) {
+    dp[i] = (char *)d + off;
+    CORAL_HIP(hipMemcpyAsync(dp[i], h[i], bytes[i], hipMemcpyHostToDevice, g_stream));
+    off += 8 * ((bytes[i] + 7) / 8);
+  }
+  int s = call((void *const *)dp, st);
+  if (s != CORAL_OK) { finish(); return s; }
+  for (int i = 0; i < count; ++i) CORAL_HIP(hipMemcpyAsync(h[i], dp[i], bytes[i], hipMemcpyDeviceToHost, g_stream));
+  return finish();
+}
+
 // reductions: result of `rbytes` copied back into *result
 template <typename F>
 static int red_host(size_t es, int64_t n, const void *x, int64_t incx, const void *y, int64_t incy, void *result,
@@ -334,5 +385,85 @@ IAMAX_HOST(isamax, float, 4)
 IAMAX_HOST(idamax, double, 8)
 IAMAX_HOST(icamax, float, 8)
 IAMAX_HOST(izamax, double, 16)
+
+#define SCAL_HOST(NAME, T, ES, S_T)                                                                              \
+  int coralh_##NAME(int64_t n, S_T alpha, T *x, int64_t incx) {                                                  \
+    return x_host(ES, n, x, incx, [&](void *dx, coral_stream_t st) { return coral_##NAME(n, alpha, (T *)dx, incx, st); }); \
+  }
+SCAL_HOST(sscal, float, 4, float)
+SCAL_HOST(dscal, double, 8, double)
+SCAL_HOST(cscal, float, 8, const float *)
+SCAL_HOST(zscal, double, 16, const double *)
+SCAL_HOST(csscal, float, 8, float)
+SCAL_HOST(zdscal, double, 16, double)
+
+NRM2_HOST(sasum, float, 4, float)
+NRM2_HOST(dasum, double, 8, double)
+NRM2_HOST(scasum, float, 8, float)
+NRM2_HOST(dzasum, double, 16, double)
+
+#define COPY_HOST(NAME, T, ES)                                                                                   \
+  int coralh_##NAME(int64_t n, const T *x, int64_t incx, T *y, int64_t incy) {                                 \
+    return v2_host(ES, n, x, incx, y, incy, true, [&](const void *dx, void *dy, coral_stream_t st) {             \
+      return coral_##NAME(n, (const T *)dx, incx, (T *)dy, incy, st);                                            \
+    });                                                                                                          \
+  }
+COPY_HOST(scopy, float, 4)
+COPY_HOST(dcopy, double, 8)
+COPY_HOST(ccopy, float, 8)
+COPY_HOST(zcopy, double, 16)
+
+#define SWAP_HOST(NAME, T, ES)                                                                                   \
+  int coralh_##NAME(int64_t n, T *x, int64_t incx, T *y, int64_t incy) {                                       \
+    return xy_host(ES, n, x, incx, y, incy, [&](void *dx, void *dy, coral_stream_t st) {                         \
+      return coral_##NAME(n, (T *)dx, incx, (T *)dy, incy, st);                                                  \
+    });                                                                                                          \
+  }
+SWAP_HOST(sswap, float, 4)
+SWAP_HOST(dswap, double, 8)
+SWAP_HOST(cswap, float, 8)
+SWAP_HOST(zswap, double, 16)
+
+#define ROT_HOST(NAME, T, ES)                                                                                    \
+  int coralh_##NAME(int64_t n, T *x, int64_t incx, T *y, int64_t incy, T c, T s) {                             \
+    return xy_host(ES, n, x, incx, y, incy, [&](void *dx, void *dy, coral_stream_t st) {                         \
+      return coral_##NAME(n, (T *)dx, incx, (T *)dy, incy, c, s, st);                                            \
+    });                                                                                                          \
+  }
+ROT_HOST(srot, float, 4)
+ROT_HOST(drot, double, 8)
+ROT_HOST(csrot, float, 8)
+ROT_HOST(zdrot, double, 16)
+
+#define ROTM_HOST(NAME, T, ES)                                                                                   \
+  int coralh_##NAME(int64_t n, T *x, int64_t incx, T *y, int64_t incy, const T param[5]) {                     \
+    return xy_host(ES, n, x, incx, y, incy, [&](void *dx, void *dy, coral_stream_t st) {                         \
+      return coral_##NAME(n, (T *)dx, incx, (T *)dy, incy, param, st);                                           \
+    });                                                                                                          \
+  }
+ROTM_HOST(srotm, float, 4)
+ROTM_HOST(drotm, double, 8)
+
+#define ROTG_HOST(NAME, T)                                                                                       \
+  int coralh_##NAME(T *a, T *b, T *c, T *s) {                                                                    \
+    void *h[4] = {a, b, c, s};                                                                                   \
+    const size_t bytes[4] = {sizeof(T), sizeof(T), sizeof(T), sizeof(T)};                                        \
+    return scalars_host(h, bytes, 4, [&](void *const *d, coral_stream_t st) {                                    \
+      return coral_##NAME((T *)d[0], (T *)d[1], (T *)d[2], (T *)d[3], st);                                       \
+    });                                                                                                          \
+  }
+ROTG_HOST(srotg, float)
+ROTG_HOST(drotg, double)
+
+#define ROTMG_HOST(NAME, T)                                                                                      \
+  int coralh_##NAME(T *d1, T *d2, T *x1, T y1, T param[5]) {                                                     \
+    void *h[4] = {d1, d2, x1, param};                                                                            \
+    const size_t bytes[4] = {sizeof(T), sizeof(T), sizeof(T), 5 * sizeof(T)};                                    \
+    return scalars_host(h, bytes, 4, [&](void *const *d, coral_stream_t st) {                                    \
+      return coral_##NAME((T *)d[0], (T *)d[1], (T *)d[2], y1, (T *)d[3], st);                                   \
+    });                                                                                                          \
+  }
+ROTMG_HOST(srotmg, float)
+ROTMG_HOST(drotmg, double)
 
 }  // extern "C"

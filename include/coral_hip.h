@@ -304,6 +304,53 @@ int coral_sscal(int64_t n, float alpha, float *x, int64_t incx, coral_stream_t s
 int coral_dscal(int64_t n, double alpha, double *x, int64_t incx, coral_stream_t stream);
 int coral_cscal(int64_t n, const float alpha[2], float *x, int64_t incx, coral_stream_t stream);
 int coral_zscal(int64_t n, const double alpha[2], double *x, int64_t incx, coral_stream_t stream);
+/* coral-aarch64/src/level1/csscal.rs:39 / zdscal.rs:38: complex x times a real alpha.
+ * All six scal routines round as CORAL does (complex scal: fused in the NEON body of
+ * a unit-stride call, every product rounded elsewhere), so results are bit-identical. */
+int coral_csscal(int64_t n, float alpha, float *x, int64_t incx, coral_stream_t stream);
+int coral_zdscal(int64_t n, double alpha, double *x, int64_t incx, coral_stream_t stream);
+
+/* coral-aarch64/src/level1/sasum.rs:40 / dasum.rs:41 / scasum.rs:40 / dzasum.rs:40:
+ * sum of |x_i| (complex |re| + |im|); n == 0 -> 0.  Result in device memory. */
+int coral_sasum(int64_t n, const float *x, int64_t incx, float *result, coral_stream_t stream);
+int coral_dasum(int64_t n, const double *x, int64_t incx, double *result, coral_stream_t stream);
+int coral_scasum(int64_t n, const float *x, int64_t incx, float *result, coral_stream_t stream);
+int coral_dzasum(int64_t n, const double *x, int64_t incx, double *result, coral_stream_t stream);
+
+/* coral-aarch64/src/level1/scopy.rs:28 / dcopy.rs:28 / ccopy.rs:29 / zcopy.rs:28: y := x */
+int coral_scopy(int64_t n, const float *x, int64_t incx, float *y, int64_t incy, coral_stream_t stream);
+int coral_dcopy(int64_t n, const double *x, int64_t incx, double *y, int64_t incy, coral_stream_t stream);
+int coral_ccopy(int64_t n, const float *x, int64_t incx, float *y, int64_t incy, coral_stream_t stream);
+int coral_zcopy(int64_t n, const double *x, int64_t incx, double *y, int64_t incy, coral_stream_t stream);
+/* coral-aarch64/src/level1/sswap.rs:35 / dswap.rs:35 / cswap.rs:36 / zswap.rs:35: x <-> y */
+int coral_sswap(int64_t n, float *x, int64_t incx, float *y, int64_t incy, coral_stream_t stream);
+int coral_dswap(int64_t n, double *x, int64_t incx, double *y, int64_t incy, coral_stream_t stream);
+int coral_cswap(int64_t n, float *x, int64_t incx, float *y, int64_t incy, coral_stream_t stream);
+int coral_zswap(int64_t n, double *x, int64_t incx, double *y, int64_t incy, coral_stream_t stream);
+
+/* coral-aarch64/src/level1/srot.rs:51 / drot.rs:48 / csrot.rs:50 / zdrot.rs:51:
+ * x' = c*x + s*y, y' = c*y - s*x (complex vectors, real c and s); bit-identical to CORAL. */
+int coral_srot(int64_t n, float *x, int64_t incx, float *y, int64_t incy, float c, float s, coral_stream_t stream);
+int coral_drot(int64_t n, double *x, int64_t incx, double *y, int64_t incy, double c, double s,
+               coral_stream_t stream);
+int coral_csrot(int64_t n, float *x, int64_t incx, float *y, int64_t incy, float c, float s, coral_stream_t stream);
+int coral_zdrot(int64_t n, double *x, int64_t incx, double *y, int64_t incy, double c, double s,
+                coral_stream_t stream);
+/* coral-aarch64/src/level1/srotm.rs:47 / drotm.rs:47: modified rotation; param is a HOST
+ * array [flag, h11, h21, h12, h22] (the flag selects the kernel); bit-identical to CORAL. */
+int coral_srotm(int64_t n, float *x, int64_t incx, float *y, int64_t incy, const float param[5],
+                coral_stream_t stream);
+int coral_drotm(int64_t n, double *x, int64_t incx, double *y, int64_t incy, const double param[5],
+                coral_stream_t stream);
+/* coral-aarch64/src/level1/srotg.rs:40 / drotg.rs:40: Givens construction on DEVICE
+ * scalars: a := r, b := z, c, s written. */
+int coral_srotg(float *a, float *b, float *c, float *s, coral_stream_t stream);
+int coral_drotg(double *a, double *b, double *c, double *s, coral_stream_t stream);
+/* coral-aarch64/src/level1/srotmg.rs:42 / drotmg.rs:41: modified Givens construction on
+ * DEVICE scalars d1, d2, x1 (in/out) and param[5] (out); y1 by value.  The GAM rescaling
+ * runs only on finite d1/d2 (CORAL's loop does not end on an infinite one). */
+int coral_srotmg(float *d1, float *d2, float *x1, float y1, float *param, coral_stream_t stream);
+int coral_drotmg(double *d1, double *d2, double *x1, double y1, double *param, coral_stream_t stream);
 
 /* ======================================================================== */
 /* Host-pointer drop-in forms: same arguments, host memory, synchronous.    */
@@ -391,6 +438,34 @@ int coralh_isamax(int64_t n, const float *x, int64_t incx, int64_t *result);
 int coralh_idamax(int64_t n, const double *x, int64_t incx, int64_t *result);
 int coralh_icamax(int64_t n, const float *x, int64_t incx, int64_t *result);
 int coralh_izamax(int64_t n, const double *x, int64_t incx, int64_t *result);
+int coralh_sscal(int64_t n, float alpha, float *x, int64_t incx);
+int coralh_dscal(int64_t n, double alpha, double *x, int64_t incx);
+int coralh_cscal(int64_t n, const float alpha[2], float *x, int64_t incx);
+int coralh_zscal(int64_t n, const double alpha[2], double *x, int64_t incx);
+int coralh_csscal(int64_t n, float alpha, float *x, int64_t incx);
+int coralh_zdscal(int64_t n, double alpha, double *x, int64_t incx);
+int coralh_sasum(int64_t n, const float *x, int64_t incx, float *result);
+int coralh_dasum(int64_t n, const double *x, int64_t incx, double *result);
+int coralh_scasum(int64_t n, const float *x, int64_t incx, float *result);
+int coralh_dzasum(int64_t n, const double *x, int64_t incx, double *result);
+int coralh_scopy(int64_t n, const float *x, int64_t incx, float *y, int64_t incy);
+int coralh_dcopy(int64_t n, const double *x, int64_t incx, double *y, int64_t incy);
+int coralh_ccopy(int64_t n, const float *x, int64_t incx, float *y, int64_t incy);
+int coralh_zcopy(int64_t n, const double *x, int64_t incx, double *y, int64_t incy);
+int coralh_sswap(int64_t n, float *x, int64_t incx, float *y, int64_t incy);
+int coralh_dswap(int64_t n, double *x, int64_t incx, double *y, int64_t incy);
+int coralh_cswap(int64_t n, float *x, int64_t incx, float *y, int64_t incy);
+int coralh_zswap(int64_t n, double *x, int64_t incx, double *y, int64_t incy);
+int coralh_srot(int64_t n, float *x, int64_t incx, float *y, int64_t incy, float c, float s);
+int coralh_drot(int64_t n, double *x, int64_t incx, double *y, int64_t incy, double c, double s);
+int coralh_csrot(int64_t n, float *x, int64_t incx, float *y, int64_t incy, float c, float s);
+int coralh_zdrot(int64_t n, double *x, int64_t incx, double *y, int64_t incy, double c, double s);
+int coralh_srotm(int64_t n, float *x, int64_t incx, float *y, int64_t incy, const float param[5]);
+int coralh_drotm(int64_t n, double *x, int64_t incx, double *y, int64_t incy, const double param[5]);
+int coralh_srotg(float *a, float *b, float *c, float *s);
+int coralh_drotg(double *a, double *b, double *c, double *s);
+int coralh_srotmg(float *d1, float *d2, float *x1, float y1, float param[5]);
+int coralh_drotmg(double *d1, double *d2, double *x1, double y1, double param[5]);
 
 #ifdef __cplusplus
 }

@@ -893,3 +893,348 @@ DEFINE_IAMAX(isamax, float, 0)
 DEFINE_IAMAX(idamax, double, 0)
 DEFINE_IAMAX(icamax, float, 1)
 DEFINE_IAMAX(izamax, double, 1)
+
+/* ======================================================================== */
+/* Level 1: the rest of the call surface                                    */
+/* ======================================================================== */
+/* ARM ADDV order of a 4-lane float vector: (l0 + l1) + (l2 + l3).           */
+static float hsum4(const float *v) { return (v[0] + v[1]) + (v[2] + v[3]); }
+
+/* sasum.rs:40-107 (real) and scasum.rs:40-108 (complex, over the 2n floats of */
+/* a unit-stride vector): four 4-lane accumulators over 16-float chunks,     */
+/* 4-float steps into acc0, res += h(acc0) + h(acc1) + h(acc2) + h(acc3),    */
+/* scalar tail; strided: res += |x| (complex: res += |re| + |im|).           */
+static float sasum_unit(int64_t len, const float *x) {
+  float acc[4][4];
+  memset(acc, 0, sizeof(acc));
+  int64_t i = 0;
+  for (; i + 16 <= len; i += 16)
+    for (int v = 0; v < 4; ++v)
+      for (int l = 0; l < 4; ++l) acc[v][l] += fabsf(x[i + 4 * v + l]);
+  for (; i + 4 <= len; i += 4)
+    for (int l = 0; l < 4; ++l) acc[0][l] += fabsf(x[i + l]);
+  float res = 0;
+  res += ((hsum4(acc[0]) + hsum4(acc[1])) + hsum4(acc[2])) + hsum4(acc[3]);
+  for (; i < len; ++i) res += fabsf(x[i]);
+  return res;
+}
+float port_sasum(int64_t n, const float *x, int64_t incx) {
+  if (n == 0 || incx == 0) return 0;
+  if (incx == 1) return sasum_unit(n, x);
+  float res = 0;
+  for (int64_t i = 0; i < n; ++i) res += fabsf(x[i * incx]);
+  return res;
+}
+float port_scasum(int64_t n, const float *x, int64_t incx) {
+  if (n == 0 || incx == 0) return 0;
+  if (incx == 1) return sasum_unit(2 * n, x);
+  float res = 0;
+  for (int64_t i = 0; i < n; ++i) res += fabsf(x[2 * i * incx]) + fabsf(x[2 * i * incx + 1]);
+  return res;
+}
+/* dasum.rs:41-110: four 2-lane accumulators over 8-element chunks, then     */
+/* 4-element steps into acc0 and acc1, tail.                                 */
+double port_dasum(int64_t n, const double *x, int64_t incx) {
+  if (n == 0 || incx == 0) return 0;
+  double res = 0;
+  if (incx == 1) {
+    double acc[4][2];
+    memset(acc, 0, sizeof(acc));
+    int64_t i = 0;
+    for (; i + 8 <= n; i += 8)
+      for (int v = 0; v < 4; ++v)
+        for (int l = 0; l < 2; ++l) acc[v][l] += fabs(x[i + 2 * v + l]);
+    for (; i + 4 <= n; i += 4)
+      for (int v = 0; v < 2; ++v)
+        for (int l = 0; l < 2; ++l) acc[v][l] += fabs(x[i + 2 * v + l]);
+    res += (((acc[0][0] + acc[0][1]) + (acc[1][0] + acc[1][1])) + (acc[2][0] + acc[2][1])) + (acc[3][0] + acc[3][1]);
+    for (; i < n; ++i) res += fabs(x[i]);
+    return res;
+  }
+  for (int64_t i = 0; i < n; ++i) res += fabs(x[i * incx]);
+  return res;
+}
+/* dzasum.rs:40-111: over the 2n doubles, 8-double chunks into four 2-lane   */
+/* accumulators, then 2-double steps into acc0, tail.                        */
+double port_dzasum(int64_t n, const double *x, int64_t incx) {
+  if (n == 0 || incx == 0) return 0;
+  double res = 0;
+  if (incx == 1) {
+    const int64_t len = 2 * n;
+    double acc[4][2];
+    memset(acc, 0, sizeof(acc));
+    int64_t i = 0;
+    for (; i + 8 <= len; i += 8)
+      for (int v = 0; v < 4; ++v)
+        for (int l = 0; l < 2; ++l) acc[v][l] += fabs(x[i + 2 * v + l]);
+    for (; i + 2 <= len; i += 2)
+      for (int l = 0; l < 2; ++l) acc[0][l] += fabs(x[i + l]);
+    res += (((acc[0][0] + acc[0][1]) + (acc[1][0] + acc[1][1])) + (acc[2][0] + acc[2][1])) + (acc[3][0] + acc[3][1]);
+    for (; i < len; ++i) res += fabs(x[i]);
+    return res;
+  }
+  for (int64_t i = 0; i < n; ++i) res += fabs(x[2 * i * incx]) + fabs(x[2 * i * incx + 1]);
+  return res;
+}
+
+/* scopy.rs:28-60 .. zcopy.rs, sswap.rs:35-120 .. zswap.rs: element moves    */
+/* (W = 1 real, 2 complex floats per element).                               */
+#define DEFINE_COPY_SWAP(P, T, W)                                                                            \
+  void port_##P##copy(int64_t n, const T *x, int64_t incx, T *y, int64_t incy) {                            \
+    for (int64_t i = 0; i < n; ++i)                                                                          \
+      for (int w = 0; w < W; ++w) y[W * i * incy + w] = x[W * i * incx + w];                                 \
+  }                                                                                                          \
+  void port_##P##swap(int64_t n, T *x, int64_t incx, T *y, int64_t incy) {                                  \
+    for (int64_t i = 0; i < n; ++i)                                                                          \
+      for (int w = 0; w < W; ++w) {                                                                          \
+        T t = x[W * i * incx + w];                                                                           \
+        x[W * i * incx + w] = y[W * i * incy + w];                                                           \
+        y[W * i * incy + w] = t;                                                                             \
+      }                                                                                                      \
+  }
+DEFINE_COPY_SWAP(s, float, 1)
+DEFINE_COPY_SWAP(d, double, 1)
+DEFINE_COPY_SWAP(c, float, 2)
+DEFINE_COPY_SWAP(z, double, 2)
+
+/* srot.rs:51-180 / drot.rs:48-178 (real) and csrot.rs:50-166 / zdrot.rs:     */
+/* 51-166 (complex vectors, real c and s; the unit-stride path works on the  */
+/* 2n floats).  Unit strides: a NEON body over the first floor(len/Q)*Q      */
+/* values, Q = 4 srot / csrot, 2 drot / zdrot, with x' = fma(y, s, c*x) and  */
+/* (real) y' = fma(-s, x, c*y) or (complex) y' = c*y - s*x; tail and strided */
+/* path: x' = c*x + s*y, y' = c*y - s*x, every product rounded.              */
+#define DEFINE_ROT(NAME, T, FMA, Q, W, YFUSED)                                                               \
+  void port_##NAME(int64_t n, T *x, int64_t incx, T *y, int64_t incy, T c, T s) {                           \
+    if (n == 0) return;                                                                                      \
+    const int unit = incx == 1 && incy == 1;                                                                 \
+    const int64_t len = W * n, nf = unit ? len / Q * Q : 0;                                                  \
+    for (int64_t k = 0; k < len; ++k) {                                                                      \
+      const int64_t i = k / W, w = k % W;                                                                    \
+      T *px = x + W * i * incx + w, *py = y + W * i * incy + w;                                              \
+      const T xv = *px, yv = *py;                                                                            \
+      if (k < nf) {                                                                                          \
+        *px = FMA(yv, s, c * xv);                                                                            \
+        *py = YFUSED ? FMA(-s, xv, c * yv) : c * yv - s * xv;                                                \
+      } else {                                                                                               \
+        *px = c * xv + s * yv;                                                                               \
+        *py = c * yv - s * xv;                                                                               \
+      }                                                                                                      \
+    }                                                                                                        \
+  }
+DEFINE_ROT(srot, float, fmaf, 4, 1, 1)
+DEFINE_ROT(drot, double, fma, 2, 1, 1)
+DEFINE_ROT(csrot, float, fmaf, 4, 2, 0)
+DEFINE_ROT(zdrot, double, fma, 2, 2, 0)
+
+/* srotm.rs:47-412 / drotm.rs:47-410: flag = param[0]; -2 is the identity;   */
+/* flag < 0: H = [h11 h12; h21 h22], flag == 0: [1 h12; h21 1], otherwise:   */
+/* [h11 1; -1 h22].  Unit strides: NEON body over floor(n/Q)*Q elements (Q = */
+/* 4 s, 2 d) with x' = fma(y, h12, h11*x), y' = fma(x, h21, h22*y) (flag<0), */
+/* x' = fma(y, h12, x), y' = fma(x, h21, y) (flag 0), x' = fma(x, h11, y),    */
+/* y' = h22*y - x (otherwise); tail and strided: every product rounded.      */
+#define DEFINE_ROTM(NAME, T, FMA, Q)                                                                         \
+  void port_##NAME(int64_t n, T *x, int64_t incx, T *y, int64_t incy, const T *param) {                     \
+    if (n == 0) return;                                                                                      \
+    const T flag = param[0], h11 = param[1], h21 = param[2], h12 = param[3], h22 = param[4];                 \
+    if (flag == (T)-2) return;                                                                               \
+    const int64_t nf = (incx == 1 && incy == 1) ? n / Q * Q : 0;                                             \
+    for (int64_t i = 0; i < n; ++i) {                                                                        \
+      T *px = x + i * incx, *py = y + i * incy;                                                              \
+      const T xv = *px, yv = *py;                                                                            \
+      T xn, yn;                                                                                              \
+      if (flag < 0) {                                                                                        \
+        xn = i < nf ? FMA(yv, h12, h11 * xv) : h11 * xv + h12 * yv;                                          \
+        yn = i < nf ? FMA(xv, h21, h22 * yv) : h21 * xv + h22 * yv;                                          \
+      } else if (flag == 0) {                                                                                \
+        xn = i < nf ? FMA(yv, h12, xv) : xv + h12 * yv;                                                      \
+        yn = i < nf ? FMA(xv, h21, yv) : h21 * xv + yv;                                                      \
+      } else {                                                                                               \
+        xn = i < nf ? FMA(xv, h11, yv) : h11 * xv + yv;                                                      \
+        yn = i < nf ? h22 * yv - xv : -xv + h22 * yv;                                                        \
+      }                                                                                                      \
+      *py = yn;                                                                                              \
+      *px = xn;                                                                                              \
+    }                                                                                                        \
+  }
+DEFINE_ROTM(srotm, float, fmaf, 4)
+DEFINE_ROTM(drotm, double, fma, 2)
+
+/* srotg.rs:40-75 / drotg.rs:40-71: r = scale*sqrt((a/scale)^2 + (b/scale)^2) */
+/* with scale = |a| + |b| and the sign of the larger of a, b; c = a/r,       */
+/* s = b/r; z = s if |a| > |b|, 1/c if |b| >= |a| and c != 0, else 1;        */
+/* a := r, b := z.  scale == 0 gives c = 1, s = 0, a = b = 0.                */
+#define DEFINE_ROTG(NAME, T, ABS, SQRT)                                                                      \
+  void port_##NAME(T *a, T *b, T *c, T *s) {                                                                 \
+    const T roe = ABS(*a) > ABS(*b) ? *a : *b;                                                               \
+    const T scale = ABS(*a) + ABS(*b);                                                                       \
+    if (scale == 0) {                                                                                        \
+      *c = 1;                                                                                                \
+      *s = 0;                                                                                                \
+      *a = 0;                                                                                                \
+      *b = 0;                                                                                                \
+      return;                                                                                                \
+    }                                                                                                        \
+    const T as = *a / scale, bs = *b / scale;                                                                \
+    T r = scale * SQRT(as * as + bs * bs);                                                                   \
+    if (roe < 0) r = -r;                                                                                     \
+    *c = *a / r;                                                                                             \
+    *s = *b / r;                                                                                             \
+    T z = 1;                                                                                                 \
+    if (ABS(*a) > ABS(*b)) z = *s;                                                                           \
+    if (ABS(*b) >= ABS(*a) && *c != 0) z = 1 / *c;                                                           \
+    *a = r;                                                                                                  \
+    *b = z;                                                                                                  \
+  }
+DEFINE_ROTG(srotg, float, fabsf, sqrtf)
+DEFINE_ROTG(drotg, double, fabs, sqrt)
+
+/* srotmg.rs:42-199 / drotmg.rs:41-195: the modified Givens construction,    */
+/* GAM = 4096 rescaling of d1 and d2 into [1/GAM^2, GAM^2].  The rescaling   */
+/* loops run only on finite d1/d2 (CORAL's loops would not end on an         */
+/* infinite one).                                                            */
+#define DEFINE_ROTMG(NAME, T, ABS, ISFIN)                                                                    \
+  void port_##NAME(T *d1, T *d2, T *x1, T y1, T *param) {                                                    \
+    const T gam = 4096, gamsq = gam * gam, rgamsq = 1 / gamsq;                                               \
+    T flag, h11 = 0, h12 = 0, h21 = 0, h22 = 0;                                                              \
+    if (*d1 < 0) {                                                                                           \
+      flag = -1;                                                                                             \
+      *d1 = 0;                                                                                               \
+      *d2 = 0;                                                                                               \
+      *x1 = 0;                                                                                               \
+    } else {                                                                                                 \
+      const T p2 = *d2 * y1;                                                                                 \
+      if (p2 == 0) {                                                                                         \
+        param[0] = -2;                                                                                       \
+        return;                                                                                              \
+      }                                                                                                      \
+      const T p1 = *d1 * *x1, q2 = p2 * y1, q1 = p1 * *x1;                                                   \
+      if (ABS(q1) > ABS(q2)) {                                                                               \
+        h21 = -y1 / *x1;                                                                                     \
+        h12 = p2 / p1;                                                                                       \
+        const T u = 1 - h12 * h21;                                                                           \
+        if (u <= 0) {                                                                                        \
+          flag = -1;                                                                                         \
+          *d1 = 0;                                                                                           \
+          *d2 = 0;                                                                                           \
+          *x1 = 0;                                                                                           \
+          h11 = h12 = h21 = h22 = 0;                                                                         \
+        } else {                                                                                             \
+          flag = 0;                                                                                          \
+          *d1 /= u;                                                                                          \
+          *d2 /= u;                                                                                          \
+          *x1 *= u;                                                                                          \
+        }                                                                                                    \
+      } else if (q2 < 0) {                                                                                   \
+        flag = -1;                                                                                           \
+        *d1 = 0;                                                                                             \
+        *d2 = 0;                                                                                             \
+        *x1 = 0;                                                                                             \
+        h11 = h12 = h21 = h22 = 0;                                                                           \
+      } else {                                                                                               \
+        flag = 1;                                                                                            \
+        h11 = p1 / p2;                                                                                       \
+        h22 = *x1 / y1;                                                                                      \
+        const T u = 1 + h11 * h22;                                                                           \
+        const T t = *d2 / u;                                                                                 \
+        *d2 = *d1 / u;                                                                                       \
+        *d1 = t;                                                                                             \
+        *x1 = y1 * u;                                                                                        \
+      }                                                                                                      \
+      if (*d1 != 0 && ISFIN(*d1)) {                                                                          \
+        while (*d1 <= rgamsq || *d1 >= gamsq) {                                                              \
+          if (flag == 0) {                                                                                   \
+            h11 = 1;                                                                                         \
+            h22 = 1;                                                                                         \
+          } else {                                                                                           \
+            h21 = -1;                                                                                        \
+            h12 = 1;                                                                                         \
+          }                                                                                                  \
+          flag = -1;                                                                                         \
+          if (*d1 <= rgamsq) {                                                                               \
+            *d1 *= gamsq;                                                                                    \
+            *x1 /= gam;                                                                                      \
+            h11 /= gam;                                                                                      \
+            h12 /= gam;                                                                                      \
+          } else {                                                                                           \
+            *d1 /= gamsq;                                                                                    \
+            *x1 *= gam;                                                                                      \
+            h11 *= gam;                                                                                      \
+            h12 *= gam;                                                                                      \
+          }                                                                                                  \
+        }                                                                                                    \
+      }                                                                                                      \
+      if (*d2 != 0 && ISFIN(*d2)) {                                                                          \
+        while (ABS(*d2) <= rgamsq || ABS(*d2) >= gamsq) {                                                    \
+          if (flag == 0) {                                                                                   \
+            h11 = 1;                                                                                         \
+            h22 = 1;                                                                                         \
+          } else {                                                                                           \
+            h21 = -1;                                                                                        \
+            h12 = 1;                                                                                         \
+          }                                                                                                  \
+          flag = -1;                                                                                         \
+          if (ABS(*d2) <= rgamsq) {                                                                          \
+            *d2 *= gamsq;                                                                                    \
+            h21 /= gam;                                                                                      \
+            h22 /= gam;                                                                                      \
+          } else {                                                                                           \
+            *d2 /= gamsq;                                                                                    \
+            h21 *= gam;                                                                                      \
+            h22 *= gam;                                                                                      \
+          }                                                                                                  \
+        }                                                                                                    \
+      }                                                                                                      \
+    }                                                                                                        \
+    param[1] = param[2] = param[3] = param[4] = 0;                                                           \
+    if (flag < 0) {                                                                                          \
+      param[1] = h11;                                                                                        \
+      param[2] = h21;                                                                                        \
+      param[3] = h12;                                                                                        \
+      param[4] = h22;                                                                                        \
+    } else if (flag == 0) {                                                                                  \
+      param[2] = h21;                                                                                        \
+      param[3] = h12;                                                                                        \
+    } else {                                                                                                 \
+      param[1] = h11;                                                                                        \
+      param[4] = h22;                                                                                        \
+    }                                                                                                        \
+    param[0] = flag;                                                                                         \
+  }
+DEFINE_ROTMG(srotmg, float, fabsf, isfinite)
+DEFINE_ROTMG(drotmg, double, fabs, isfinite)
+
+/* sscal.rs:30-66 / dscal.rs:30-66 (x := alpha*x, one rounding),             */
+/* csscal.rs:39-88 / zdscal.rs:38-88 (real alpha, each part one rounding),   */
+/* cscal.rs:43-111 / zscal.rs:41-109: unit stride, NEON body over floor(n/Q)*Q */
+/* elements (Q = 4 c, 2 z) with re' = fma(-xi, ai, xr*ar), im' = fma(xr, ai,  */
+/* xi*ar); tail and strided re' = ar*xr - ai*xi, im' = ar*xi + ai*xr.        */
+/* Quick return: n == 0 or incx == 0.                                        */
+#define DEFINE_RSCAL(NAME, T, W)                                                                             \
+  void port_##NAME(int64_t n, T alpha, T *x, int64_t incx) {                                                 \
+    if (n == 0 || incx == 0) return;                                                                         \
+    for (int64_t i = 0; i < n; ++i)                                                                          \
+      for (int w = 0; w < W; ++w) x[W * i * incx + w] = x[W * i * incx + w] * alpha;                        \
+  }
+DEFINE_RSCAL(sscal, float, 1)
+DEFINE_RSCAL(dscal, double, 1)
+DEFINE_RSCAL(csscal, float, 2)
+DEFINE_RSCAL(zdscal, double, 2)
+#define DEFINE_CSCAL(NAME, T, FMA, Q)                                                                        \
+  void port_##NAME(int64_t n, const T *alpha, T *x, int64_t incx) {                                          \
+    if (n == 0 || incx == 0) return;                                                                         \
+    const T ar = alpha[0], ai = alpha[1];                                                                    \
+    const int64_t nf = incx == 1 ? n / Q * Q : 0;                                                            \
+    for (int64_t i = 0; i < n; ++i) {                                                                        \
+      T *p = x + 2 * i * incx;                                                                               \
+      const T xr = p[0], xi = p[1];                                                                          \
+      if (i < nf) {                                                                                          \
+        p[0] = FMA(-xi, ai, xr * ar);                                                                        \
+        p[1] = FMA(xr, ai, xi * ar);                                                                         \
+      } else {                                                                                               \
+        p[0] = ar * xr - ai * xi;                                                                            \
+        p[1] = ar * xi + ai * xr;                                                                            \
+      }                                                                                                      \
+    }                                                                                                        \
+  }
+DEFINE_CSCAL(cscal, float, fmaf, 4)
+DEFINE_CSCAL(zscal, double, fma, 2)

@@ -63,6 +63,30 @@ _SIGS["port_ssyr2"] = (None, [i32, i64, f32, P, i64, P, i64, P, i64])
 _SIGS["port_dsyr2"] = (None, [i32, i64, f64, P, i64, P, i64, P, i64])
 _SIGS["port_cher2"] = (None, [i32, i64, P, P, i64, P, i64, P, i64])
 _SIGS["port_zher2"] = (None, [i32, i64, P, P, i64, P, i64, P, i64])
+# rest of level 1
+_SIGS["port_sasum"] = (f32, [i64, P, i64])
+_SIGS["port_dasum"] = (f64, [i64, P, i64])
+_SIGS["port_scasum"] = (f32, [i64, P, i64])
+_SIGS["port_dzasum"] = (f64, [i64, P, i64])
+for _p in "sdcz":
+    _SIGS[f"port_{_p}copy"] = (None, [i64, P, i64, P, i64])
+    _SIGS[f"port_{_p}swap"] = (None, [i64, P, i64, P, i64])
+_SIGS["port_srot"] = (None, [i64, P, i64, P, i64, f32, f32])
+_SIGS["port_drot"] = (None, [i64, P, i64, P, i64, f64, f64])
+_SIGS["port_csrot"] = (None, [i64, P, i64, P, i64, f32, f32])
+_SIGS["port_zdrot"] = (None, [i64, P, i64, P, i64, f64, f64])
+_SIGS["port_srotm"] = (None, [i64, P, i64, P, i64, P])
+_SIGS["port_drotm"] = (None, [i64, P, i64, P, i64, P])
+_SIGS["port_srotg"] = (None, [P, P, P, P])
+_SIGS["port_drotg"] = (None, [P, P, P, P])
+_SIGS["port_srotmg"] = (None, [P, P, P, f32, P])
+_SIGS["port_drotmg"] = (None, [P, P, P, f64, P])
+_SIGS["port_sscal"] = (None, [i64, f32, P, i64])
+_SIGS["port_dscal"] = (None, [i64, f64, P, i64])
+_SIGS["port_csscal"] = (None, [i64, f32, P, i64])
+_SIGS["port_zdscal"] = (None, [i64, f64, P, i64])
+_SIGS["port_cscal"] = (None, [i64, P, P, i64])
+_SIGS["port_zscal"] = (None, [i64, P, P, i64])
 
 _lib = None
 
@@ -177,7 +201,8 @@ def _mk_axpy(p):
     def f(n, alpha, x, incx, y, incy):
         fn = getattr(load(), f"port_{p}axpy")
         if p in "cz":
-            fn(n, _s2(alpha, p).ctypes.data, _ptr(x), incx, _ptr(y), incy)
+            al = _s2(alpha, p)  # kept alive across the call
+            fn(n, al.ctypes.data, _ptr(x), incx, _ptr(y), incy)
         else:
             fn(n, alpha, _ptr(x), incx, _ptr(y), incy)
     return f
@@ -188,7 +213,8 @@ saxpy, daxpy, caxpy, zaxpy = (_mk_axpy(p) for p in "sdcz")
 
 def _mk_ger(p, name):
     def f(m, n, alpha, x, incx, y, incy, a, lda):
-        al = _s2(alpha, p).ctypes.data if p in "cz" else alpha
+        keep = _s2(alpha, p) if p in "cz" else None  # kept alive across the call
+        al = keep.ctypes.data if p in "cz" else alpha
         getattr(load(), f"port_{name}")(m, n, al, _ptr(x), incx, _ptr(y), incy, _ptr(a), lda)
     return f
 
@@ -209,7 +235,8 @@ ssyr, dsyr, cher, zher = (_mk_syr(nm) for nm in ("ssyr", "dsyr", "cher", "zher")
 
 def _mk_syr2(p, name):
     def f(uplo, n, alpha, x, incx, y, incy, a, lda):
-        al = _s2(alpha, p).ctypes.data if p in "cz" else alpha
+        keep = _s2(alpha, p) if p in "cz" else None  # kept alive across the call
+        al = keep.ctypes.data if p in "cz" else alpha
         getattr(load(), f"port_{name}")(int(uplo), n, al, _ptr(x), incx, _ptr(y), incy, _ptr(a), lda)
     return f
 
@@ -248,3 +275,83 @@ def icamax(n, x, incx):
 
 def izamax(n, x, incx):
     return int(load().port_izamax(n, _ptr(x), incx))
+
+
+# ---------------------------------------------------------------- rest of level 1
+def _mk_asum(name):
+    def f(n, x, incx):
+        return getattr(load(), f"port_{name}")(n, _ptr(x), incx)
+    return f
+
+
+sasum, dasum, scasum, dzasum = (_mk_asum(nm) for nm in ("sasum", "dasum", "scasum", "dzasum"))
+
+
+def _mk_xy(name):
+    def f(n, x, incx, y, incy):
+        getattr(load(), f"port_{name}")(n, _ptr(x), incx, _ptr(y), incy)
+    return f
+
+
+scopy, dcopy, ccopy, zcopy = (_mk_xy(f"{p}copy") for p in "sdcz")
+sswap, dswap, cswap, zswap = (_mk_xy(f"{p}swap") for p in "sdcz")
+
+
+def _mk_rot(name):
+    def f(n, x, incx, y, incy, c, s):
+        getattr(load(), f"port_{name}")(n, _ptr(x), incx, _ptr(y), incy, c, s)
+    return f
+
+
+srot, drot, csrot, zdrot = (_mk_rot(nm) for nm in ("srot", "drot", "csrot", "zdrot"))
+
+
+def _mk_rotm(name, real):
+    def f(n, x, incx, y, incy, param):
+        pa = np.ascontiguousarray(param, real)
+        getattr(load(), f"port_{name}")(n, _ptr(x), incx, _ptr(y), incy, pa.ctypes.data)
+    return f
+
+
+srotm, drotm = _mk_rotm("srotm", np.float32), _mk_rotm("drotm", np.float64)
+
+
+def _mk_rotg(name, real):
+    def f(a, b):
+        """Returns (r, z, c, s): CORAL's a and b on return, then c and s."""
+        v = np.array([a, b, 0, 0], real)
+        p = v.ctypes.data
+        sz = v.itemsize
+        getattr(load(), f"port_{name}")(p, p + sz, p + 2 * sz, p + 3 * sz)
+        return tuple(v)
+    return f
+
+
+srotg, drotg = _mk_rotg("srotg", np.float32), _mk_rotg("drotg", np.float64)
+
+
+def _mk_rotmg(name, real):
+    def f(d1, d2, x1, y1):
+        """Returns (d1, d2, x1, param[5]) after CORAL's rotmg."""
+        v = np.array([d1, d2, x1], real)
+        param = np.zeros(5, real)
+        p, sz = v.ctypes.data, v.itemsize
+        getattr(load(), f"port_{name}")(p, p + sz, p + 2 * sz, y1, param.ctypes.data)
+        return v[0], v[1], v[2], param
+    return f
+
+
+srotmg, drotmg = _mk_rotmg("srotmg", np.float32), _mk_rotmg("drotmg", np.float64)
+
+
+def _mk_scal(name, p):
+    def f(n, alpha, x, incx):
+        keep = _s2(alpha, p) if name in ("cscal", "zscal") else None  # kept alive across the call
+        al = keep.ctypes.data if keep is not None else alpha
+        getattr(load(), f"port_{name}")(n, al, _ptr(x), incx)
+    return f
+
+
+sscal, dscal = _mk_scal("sscal", "s"), _mk_scal("dscal", "d")
+cscal, zscal = _mk_scal("cscal", "c"), _mk_scal("zscal", "z")
+csscal, zdscal = _mk_scal("csscal", "c"), _mk_scal("zdscal", "z")

@@ -26,6 +26,10 @@ def test_header_declares_the_call_surface():
     for r in ("sger", "dger", "cgeru", "cgerc", "zgeru", "zgerc", "ssyr", "dsyr", "cher", "zher",
               "ssyr2", "dsyr2", "cher2", "zher2"):
         assert f"coral_{r}" in names and f"coralh_{r}" in names, r
+    for r in ("sscal", "dscal", "cscal", "zscal", "csscal", "zdscal", "sasum", "dasum", "scasum", "dzasum",
+              "scopy", "dcopy", "ccopy", "zcopy", "sswap", "dswap", "cswap", "zswap", "srot", "drot", "csrot",
+              "zdrot", "srotm", "drotm", "srotg", "drotg", "srotmg", "drotmg"):
+        assert f"coral_{r}" in names and f"coralh_{r}" in names, r
 
 
 def test_library_exports_every_declared_symbol():
@@ -101,6 +105,31 @@ def test_rank_update_validation_codes():
     assert _status("coral_dger", 4, 4, 0.0, p, 1, p, 1, p, 4, None) == 0
     assert _status("coral_dsyr", 0, 0, 1.0, p, 1, p, 1, None) == 0
     assert _status("coral_zher2", 1, 4, zero, p, 1, p, 1, p, 4, None) == 0
+
+
+def test_level1_rest_validation_codes():
+    buf = np.zeros(64)
+    p = buf.ctypes.data
+    pvf = np.array([-1.0, 1.0, 0.0, 0.0, 1.0], np.float32)
+    # copy/swap/rot/rotm: n=1 x=2 incx=3 y=4 incy=5
+    assert _status("coral_dcopy", -1, p, 1, p, 1, None) == -1
+    assert _status("coral_dcopy", 4, p, 0, p, 1, None) == -3
+    assert _status("coral_zswap", 4, p, 1, p, 0, None) == -5
+    assert _status("coral_drot", 4, None, 1, p, 1, 1.0, 0.0, None) == -2
+    assert _status("coral_srotm", 4, p, 1, None, 1, pvf.ctypes.data, None) == -4
+    assert _status("coral_drotm", 4, p, 1, p, 1, None, None) == -6
+    # asum: n=1 x=2 incx=3 result=4; scal: n=1 alpha=2 x=3 incx=4
+    assert _status("coral_dasum", 4, p, 0, p, None) == -3
+    assert _status("coral_dzasum", 4, p, 1, None, None) == -4
+    assert _status("coral_zdscal", 4, 2.0, p, 0, None) == -4
+    assert _status("coral_csscal", 4, 2.0, None, 1, None) == -3
+    # rotg: a=1 b=2 c=3 s=4; rotmg: d1=1 d2=2 x1=3 y1=4 param=5
+    assert _status("coral_drotg", p, p, None, p, None) == -3
+    assert _status("coral_srotmg", p, p, p, 1.0, None, None) == -5
+    # n == 0 and flag == -2 return before any device call
+    assert _status("coral_dcopy", 0, p, 1, p, 1, None) == 0
+    pid = np.array([-2.0, 0, 0, 0, 0])
+    assert _status("coral_drotm", 4, p, 1, p, 1, pid.ctypes.data, None) == 0
 
 
 def test_quick_returns_need_no_device():

@@ -127,3 +127,121 @@ def test_scal(gpu_lib):
     call("coral_dscal", 4, 0.0, d.ptr, 1, None)  # multiplies: NaN stays NaN (dscal.rs:30-66)
     r = d.to_numpy()
     assert r[0] == 0.0 and np.isnan(r[1]) and r[2] == 0.0
+
+
+# ---------------------------------------------------------------- rest of level 1
+DTP = {"s": np.float32, "d": np.float64, "c": np.complex64, "z": np.complex128}
+RP = {"s": np.float32, "d": np.float64, "c": np.float32, "z": np.float64}
+# (n, incx, incy, offset in elements): unit aligned with ragged tails, unaligned, strided
+XY_CASES = [(0, 1, 1, 0), (1, 1, 1, 0), (13, 1, 1, 0), (1024, 1, 1, 0), (1023, 1, 1, 0), (1023, 1, 1, 1),
+            (777, 3, 2, 0), (1, 5, 7, 0), (257, 2, 3, 0), (100003, 1, 1, 0)]
+
+
+def _vec(rng, n, inc, off, dt):
+    m = ((n - 1) * inc + 1 if n else 1) + off
+    v = rng.uniform(-1, 1, m)
+    if np.issubdtype(dt, np.complexfloating):
+        v = v + 1j * rng.uniform(-1, 1, m)
+    return v.astype(dt)
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint8)
+
+
+@pytest.mark.parametrize("p", "sdcz")
+def test_scal_all_forms_bit_exact(gpu_lib, p):
+    """sscal/dscal (one rounding), cscal/zscal (CORAL's fused body and rounded tail),
+    csscal/zdscal (real alpha) against the oracle, bit for bit."""
+    rng = np.random.default_rng(40)
+    forms = [(f"{p}scal", 0.7)] if p in "sd" else [(f"{p}scal", (0.75, -0.2)), (f"{p}scal", (0.0, 1.0)),
+                                                    ({"c": "csscal", "z": "zdscal"}[p], -1.25)]
+    for name, alpha in forms:
+        for n, inc, _, off in XY_CASES:
+            x = _vec(rng, n, inc, off, DTP[p])
+            g, o = x.copy(), x.copy()
+            getattr(blas, name)(n, alpha, g[off:], inc)
+            getattr(port, name)(n, alpha, o[off:], inc)
+            assert np.array_equal(_bits(g), _bits(o)), (name, n, inc, off)
+
+
+@pytest.mark.parametrize("p", "sdcz")
+def test_asum(gpu_lib, p):
+    """Within n*eps of the oracle's CORAL-order sum (the GPU sums in a tree); exact on
+    dyadic inputs, where every order gives the exact sum."""
+    rng = np.random.default_rng(41)
+    name = {"s": "sasum", "d": "dasum", "c": "scasum", "z": "dzasum"}[p]
+    eps = float(np.finfo(RP[p]).eps)
+    for n, inc, _, off in XY_CASES:
+        x = _vec(rng, n, inc, off, DTP[p])
+        got = float(getattr(blas, name)(n, x[off:], inc))
+        want = float(getattr(port, name)(n, x[off:], inc))
+        assert abs(got - want) <= 2 * max(n, 1) * eps * abs(want), (n, inc)
+    v = rng.integers(-64, 65, 20000) / 8.0
+    x = v.astype(DTP[p]) if p in "sd" else (v[0::2] + 1j * v[1::2]).astype(DTP[p])
+    assert float(getattr(blas, name)(len(x), x, 1)) == float(np.abs(v).sum())
+
+
+@pytest.mark.parametrize("p", "sdcz")
+def test_copy_swap_bit_exact(gpu_lib, p):
+    rng = np.random.default_rng(42)
+    for n, incx, incy, off in XY_CASES:
+        x, y = _vec(rng, n, incx, off, DTP[p]), _vec(rng, n, incy, off, DTP[p])
+        gy, oy = y.copy(), y.copy()
+        getattr(blas, f"{p}copy")(n, x[off:], incx, gy[off:], incy)
+        getattr(port, f"{p}copy")(n, x[off:], incx, oy[off:], incy)
+        assert np.array_equal(_bits(gy), _bits(oy)), ("copy", n, incx, incy, off)
+        gx, gy, ox, oy = x.copy(), y.copy(), x.copy(), y.copy()
+        getattr(blas, f"{p}swap")(n, gx[off:], incx, gy[off:], incy)
+        getattr(port, f"{p}swap")(n, ox[off:], incx, oy[off:], incy)
+        assert np.array_equal(_bits(gx), _bits(ox)) and np.array_equal(_bits(gy), _bits(oy)), ("swap", n)
+
+
+@pytest.mark.parametrize("p", "sdcz")
+def test_rot_bit_exact(gpu_lib, p):
+    rng = np.random.default_rng(43)
+    name = {"s": "srot", "d": "drot", "c": "csrot", "z": "zdrot"}[p]
+    for n, incx, incy, off in XY_CASES:
+        c, s = float(rng.uniform(-1, 1)), float(rng.uniform(-1, 1))
+        x, y = _vec(rng, n, incx, off, DTP[p]), _vec(rng, n, incy, off, DTP[p])
+        gx, gy, ox, oy = x.copy(), y.copy(), x.copy(), y.copy()
+        getattr(blas, name)(n, gx[off:], incx, gy[off:], incy, c, s)
+        getattr(port, name)(n, ox[off:], incx, oy[off:], incy, c, s)
+        assert np.array_equal(_bits(gx), _bits(ox)) and np.array_equal(_bits(gy), _bits(oy)), (name, n, incx)
+
+
+@pytest.mark.parametrize("p", "sd")
+def test_rotm_bit_exact(gpu_lib, p):
+    rng = np.random.default_rng(44)
+    params = [[-2.0, 1.0, 2.0, 3.0, 4.0], [-1.0, 1.1, -0.4, 0.7, 0.9], [0.0, 0.0, -0.6, 0.8, 0.0],
+              [1.0, 1.2, 0.0, 0.0, -0.75], [-1.0, -0.3, 0.5, -0.9, 0.4]]
+    for param in params:
+        for n, incx, incy, off in XY_CASES:
+            x, y = _vec(rng, n, incx, off, DTP[p]), _vec(rng, n, incy, off, DTP[p])
+            gx, gy, ox, oy = x.copy(), y.copy(), x.copy(), y.copy()
+            getattr(blas, f"{p}rotm")(n, gx[off:], incx, gy[off:], incy, param)
+            getattr(port, f"{p}rotm")(n, ox[off:], incx, oy[off:], incy, param)
+            assert np.array_equal(_bits(gx), _bits(ox)) and np.array_equal(_bits(gy), _bits(oy)), (param, n)
+
+
+@pytest.mark.parametrize("p", "sd")
+def test_rotg_rotmg_bit_exact(gpu_lib, p):
+    """The single-thread device constructions give the oracle's bits (IEEE division
+    and square root on both sides), including the zero, rescaling and infinite cases."""
+    rng = np.random.default_rng(45)
+    pairs = [(3.5, -2.25), (0.0, 5.0), (-4.0, 0.0), (-1.25, -7.5), (12.0, -0.75), (0.0, 0.0), (2.5, 2.5),
+             (1e-30, 2e-30), (7.0, -1e-3)] + [tuple(rng.uniform(-10, 10, 2)) for _ in range(40)]
+    for a, b in pairs:
+        got = getattr(blas, f"{p}rotg")(a, b)
+        want = getattr(port, f"{p}rotg")(a, b)
+        assert _bits(np.array(got)).tobytes() == _bits(np.array(want)).tobytes(), (a, b, got, want)
+    quads = [(1.5, 2.0, 0.75, -1.25), (1e-3, 3e2, -4.5, 7.25), (5.0, 0.75, -2.0, 3.0), (1e4, 1e-5, 8.0, -6.0),
+             (-1.0, 2.0, 3.0, 4.0), (4.0, 2.0, 3.0, 0.0), (1e-9, 1.0, 1.0, 1.0), (1e9, 1.0, 1.0, 1e-3),
+             (1.0, 1e-12, 1.0, 1.0), (1.0, -2.0, 0.1, 5.0), (np.inf, 1.0, 1.0, 1.0)] + \
+        [tuple(rng.uniform(0.01, 10, 4) * [1, 1, rng.choice([-1, 1]), rng.choice([-1, 1])]) for _ in range(40)]
+    for q in quads:
+        got = getattr(blas, f"{p}rotmg")(*q)
+        want = getattr(port, f"{p}rotmg")(*q)
+        gb = _bits(np.concatenate([np.array(got[:3]), got[3]]))
+        wb = _bits(np.concatenate([np.array(want[:3]), want[3]]))
+        assert gb.tobytes() == wb.tobytes(), (q, got, want)
