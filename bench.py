@@ -683,7 +683,30 @@ def suite_rows(parts=("gemv", "symv", "tr", "rank", "l1")):
         emit(f"I{p.upper()}AMAX n=2^27", ms, es * n,
              cpu_leg(lambda: getattr(port, f"i{p}amax")(hn, hx, 1), es * hn, "GB/s", 1e9, "n=2^22"))
         ms = timed_steps(lambda: call(f"coral_{p}scal", n, scal(dt, 1.0), x.ptr, 1, None), 20, 3)
-        emit(f"{p.upper()}SCAL n=2^27", ms, 2 * es * n)
+        hx2 = hx.copy()
+        emit(f"{p.upper()}SCAL n=2^27", ms, 2 * es * n,
+             cpu_leg(lambda: getattr(port, f"{p}scal")(hn, scal(dt, 1.0) if p != "z" else (1.0, 0.0), hx2, 1),
+                     2 * es * hn, "GB/s", 1e9, "n=2^22"))
+        asum = {"s": "sasum", "d": "dasum", "z": "dzasum"}[p]
+        ms = timed_steps(lambda: call(f"coral_{asum}", n, x.ptr, 1, res.ptr, None), 20, 3)
+        emit(f"{asum.upper()} n=2^27", ms, es * n,
+             cpu_leg(lambda: getattr(port, asum)(hn, hx, 1), es * hn, "GB/s", 1e9, "n=2^22"))
+        ms = timed_steps(lambda: call(f"coral_{p}copy", n, x.ptr, 1, y.ptr, 1, None), 20, 3)
+        emit(f"{p.upper()}COPY n=2^27", ms, 2 * es * n,
+             cpu_leg(lambda: getattr(port, f"{p}copy")(hn, hx, 1, hy2, 1), 2 * es * hn, "GB/s", 1e9, "n=2^22"))
+        ms = timed_steps(lambda: call(f"coral_{p}swap", n, x.ptr, 1, y.ptr, 1, None), 20, 3)
+        emit(f"{p.upper()}SWAP n=2^27", ms, 4 * es * n,
+             cpu_leg(lambda: getattr(port, f"{p}swap")(hn, hx2, 1, hy2, 1), 4 * es * hn, "GB/s", 1e9, "n=2^22"))
+        rot = {"s": "srot", "d": "drot", "z": "zdrot"}[p]
+        ms = timed_steps(lambda: call(f"coral_{rot}", n, x.ptr, 1, y.ptr, 1, 0.6, 0.8, None), 20, 3)
+        emit(f"{rot.upper()} n=2^27", ms, 4 * es * n,
+             cpu_leg(lambda: getattr(port, rot)(hn, hx2, 1, hy2, 1, 0.6, 0.8), 4 * es * hn, "GB/s", 1e9, "n=2^22"))
+        if p in "sd":
+            prm = np.array([-1.0, 0.6, -0.8, 0.8, 0.6], dt)
+            ms = timed_steps(lambda: call(f"coral_{p}rotm", n, x.ptr, 1, y.ptr, 1, prm.ctypes.data, None), 20, 3)
+            emit(f"{p.upper()}ROTM n=2^27 (full H)", ms, 4 * es * n,
+                 cpu_leg(lambda: getattr(port, f"{p}rotm")(hn, hx2, 1, hy2, 1, prm), 4 * es * hn, "GB/s", 1e9,
+                         "n=2^22"))
         del x, y
 
 
@@ -694,7 +717,7 @@ def run_suite(which):
     if lib.coral_init() != 0:
         raise SystemExit(f"coral_init failed: {lib.coral_last_error().decode()}")
     table = {"c1": suite_c1, "c2": suite_c2, "c3": suite_c3, "c4": suite_c4, "c5": suite_c5, "rows": suite_rows,
-             "rank": lambda: suite_rows(("rank",))}
+             "rank": lambda: suite_rows(("rank",)), "l1": lambda: suite_rows(("l1",))}
     for w in which:
         log(f"suite {w} ...")
         table[w]()
