@@ -717,7 +717,7 @@ def run_suite(which):
     if lib.coral_init() != 0:
         raise SystemExit(f"coral_init failed: {lib.coral_last_error().decode()}")
     table = {"c1": suite_c1, "c2": suite_c2, "c3": suite_c3, "c4": suite_c4, "c5": suite_c5, "rows": suite_rows,
-             "rank": lambda: suite_rows(("rank",)), "l1": lambda: suite_rows(("l1",))}
+             **{part: (lambda part=part: suite_rows((part,))) for part in ("gemv", "symv", "tr", "rank", "l1")}}
     for w in which:
         log(f"suite {w} ...")
         table[w]()
@@ -732,7 +732,7 @@ def main():
     ap.add_argument("--gemv-n", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--suite", default="",
-                    help="comma list of c1,c2,c3,c4,c5,rows,rank: one JSON line per configuration")
+                    help="comma list of c1,c2,c3,c4,c5,rows (or one part of rows: gemv,symv,tr,rank,l1): one JSON line per configuration")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:

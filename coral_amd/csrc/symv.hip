@@ -13,6 +13,13 @@
 // Every row block receives exactly nb partials (one per partner block), stored in
 // Q[r][c][NB]; a second kernel adds them in c order and applies alpha/beta
 // (beta == 0 overwrites y, as chemv.rs:135-139 does).
+//
+// TRMV (x := op(A) x, A triangular; coral-aarch64/src/level2/dtrmv.rs:58,
+// dtrlmv.rs:47-469, dtrumv.rs:48-460, ctrmv.rs, ztrmv.rs) streams the stored
+// triangle the same way and keeps one of the two products: op = N takes the
+// block's rows (partial of row block r from partner c), op = T / C its columns
+// (partial of row block c from partner r).  A row block's partners are then
+// the blocks on one side of the diagonal, which the reduce adds in order.
 #include "coral_common.hpp"
 
 #include <algorithm>
@@ -39,31 +46,19 @@ __device__ __forceinline__ void block_of(int64_t t, int64_t &lo, int64_t &hi) {
 
 typedef uint32_t sv4u __attribute__((ext_vector_type(4)));
 
-// One stored NB x NB block per workgroup.  The block is streamed into LDS with
-// 16-byte nontemporal loads (each half-wave or wave covers one contiguous column
-// run), then u = A_blk x_c (by rows) and v = A_blk^H x_r (by columns) are formed
-// from LDS by four row/column segments each and combined in segment order.
-template <typename E, bool HERM>
-__global__ __launch_bounds__(256) void symv_block_kernel(int64_t n, int upper, const E *__restrict__ A, int64_t lda,
-                                                         const E *__restrict__ x, int64_t incx, E *__restrict__ Q,
-                                                         int64_t nb, int vec_ok) {
-  constexpr int EV = 16 / sizeof(E);         // elements per 16-byte vector
-  constexpr int VPC = NB / EV;               // vectors per block column
-  constexpr int LDP = NB + 1;                // padded column length: column reads hit distinct banks
-  __shared__ E s[NB * LDP];                  // s[c * LDP + r]
-  __shared__ E xr_s[NB], xc_s[NB];
-  __shared__ E up[4][NB], vp[4][NB];
-  int64_t lo, hi;
-  block_of(blockIdx.x, lo, hi);
-  // stored block (br, bc).  Consecutive workgroups take consecutive row blocks of
-  // one block column (upper: rows 0..hi of column hi; lower: the mirror image), so
-  // concurrently running workgroups read neighbouring runs of the same columns.
-  const int64_t br = upper ? lo : nb - 1 - lo, bc = upper ? hi : nb - 1 - hi;
-  const int64_t r0 = br * NB, c0 = bc * NB;
-  const int t = threadIdx.x;
-  const bool diag = br == bc;
-  // ---- stream the block into LDS: all loads issued before the first store ----
+// Streams the NB x NB block at (r0, c0) into LDS, s[c * (NB + 1) + r], with all of
+// the block's loads issued before the first LDS store: 16-byte nontemporal loads
+// when the block lies inside the matrix and the columns are 16-byte aligned (each
+// half-wave or wave covers one contiguous column run), element loads with zero
+// fill otherwise.
+template <typename E>
+__device__ __forceinline__ void stream_block(const E *__restrict__ A, int64_t lda, int64_t n, int64_t r0, int64_t c0,
+                                             int vec_ok, E *s) {
+  constexpr int EV = 16 / sizeof(E);  // elements per 16-byte vector
+  constexpr int VPC = NB / EV;        // vectors per block column
+  constexpr int LDP = NB + 1;         // padded column length: column reads hit distinct banks
   constexpr int IT = NB * VPC / 256;  // vectors per thread
+  const int t = threadIdx.x;
   const bool full = vec_ok && c0 + NB <= n && r0 + NB <= n;
   E blk[IT][EV];
   if (full) {
@@ -91,6 +86,29 @@ __global__ __launch_bounds__(256) void symv_block_kernel(int64_t n, int upper, c
 #pragma unroll
     for (int e = 0; e < EV; ++e) s[c * LDP + rv + e] = blk[it][e];
   }
+}
+
+// One stored NB x NB block per workgroup, streamed into LDS; then u = A_blk x_c
+// (by rows) and v = A_blk^H x_r (by columns) are formed from LDS by four row/column
+// segments each and combined in segment order.
+template <typename E, bool HERM>
+__global__ __launch_bounds__(256) void symv_block_kernel(int64_t n, int upper, const E *__restrict__ A, int64_t lda,
+                                                         const E *__restrict__ x, int64_t incx, E *__restrict__ Q,
+                                                         int64_t nb, int vec_ok) {
+  constexpr int LDP = NB + 1;
+  __shared__ E s[NB * LDP];                  // s[c * LDP + r]
+  __shared__ E xr_s[NB], xc_s[NB];
+  __shared__ E up[4][NB], vp[4][NB];
+  int64_t lo, hi;
+  block_of(blockIdx.x, lo, hi);
+  // stored block (br, bc).  Consecutive workgroups take consecutive row blocks of
+  // one block column (upper: rows 0..hi of column hi; lower: the mirror image), so
+  // concurrently running workgroups read neighbouring runs of the same columns.
+  const int64_t br = upper ? lo : nb - 1 - lo, bc = upper ? hi : nb - 1 - hi;
+  const int64_t r0 = br * NB, c0 = bc * NB;
+  const int t = threadIdx.x;
+  const bool diag = br == bc;
+  stream_block(A, lda, n, r0, c0, vec_ok, s);
   if (t < NB) {
     const int64_t i = r0 + t;
     xr_s[t] = i < n ? x[i * incx] : ezero<E>();
@@ -225,6 +243,130 @@ static int symv_impl(int32_t uplo, int64_t n, E alpha, const E *A, int64_t lda, 
   return CORAL_OK;
 }
 
+// ---- TRMV ----
+// OP 0: y = A x (rows of the block times x_c); OP 1: A^T x, OP 2: A^H x (columns
+// times x_r).  In a diagonal block only the stored half counts; a unit diagonal
+// contributes x itself and its stored value is never used.
+template <typename E, int OP>
+__global__ __launch_bounds__(256) void trmv_block_kernel(int64_t n, int upper, int unit, const E *__restrict__ A,
+                                                         int64_t lda, const E *__restrict__ x, int64_t incx,
+                                                         E *__restrict__ Q, int64_t nb, int vec_ok) {
+  constexpr int LDP = NB + 1;
+  __shared__ E s[NB * LDP];
+  __shared__ E xs[NB];
+  __shared__ E part[4][NB];
+  int64_t lo, hi;
+  block_of(blockIdx.x, lo, hi);
+  const int64_t br = upper ? lo : nb - 1 - lo, bc = upper ? hi : nb - 1 - hi;
+  const int64_t r0 = br * NB, c0 = bc * NB;
+  const int t = threadIdx.x;
+  const bool diag = br == bc;
+  stream_block(A, lda, n, r0, c0, vec_ok, s);
+  if (t < NB) {
+    const int64_t j = (OP == 0 ? c0 : r0) + t;
+    xs[t] = j < n ? x[j * incx] : ezero<E>();
+  }
+  __syncthreads();
+  constexpr int SEG = NB / 4;
+  const int idx = t % NB, seg = t / NB, k0 = seg * SEG;
+  E acc = ezero<E>();
+  if (!diag) {
+#pragma unroll
+    for (int kk = 0; kk < SEG; ++kk) {
+      const int k = k0 + kk;
+      acc = emad<OP == 2>(acc, OP == 0 ? s[k * LDP + idx] : s[idx * LDP + k], xs[k]);
+    }
+  } else {
+#pragma unroll
+    for (int kk = 0; kk < SEG; ++kk) {
+      const int k = k0 + kk;
+      const int row = OP == 0 ? idx : k, col = OP == 0 ? k : idx;
+      if (row == col) {
+        acc = unit ? eadd(acc, xs[k]) : emad<OP == 2>(acc, s[k * LDP + k], xs[k]);
+      } else if (upper ? row < col : row > col) {
+        acc = emad<OP == 2>(acc, OP == 0 ? s[k * LDP + idx] : s[idx * LDP + k], xs[k]);
+      }
+    }
+  }
+  part[seg][idx] = acc;
+  __syncthreads();
+  if (t < NB) {
+    const E sum = eadd(eadd(part[0][t], part[1][t]), eadd(part[2][t], part[3][t]));
+    const int64_t dest = OP == 0 ? br : bc, partner = OP == 0 ? bc : br;
+    Q[(dest * nb + partner) * NB + t] = sum;
+  }
+}
+
+// x[i] := sum of row block r's partials over its partners, which are c = 0..r
+// (from_zero) or c = r..nb-1.  A workgroup takes one row block (64 rows); its four
+// wavefronts sum consecutive quarters of the partner range in order and the
+// quarter sums are added in order, so the result does not depend on scheduling.
+template <typename E>
+__global__ __launch_bounds__(256) void trmv_reduce_kernel(int64_t n, int64_t nb, const E *__restrict__ Q, int from_zero,
+                                                          E *x, int64_t incx) {
+  __shared__ E part[4][64];
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int64_t r = blockIdx.x, i = r * NB + lane;
+  const int64_t lo = from_zero ? 0 : r, cnt = from_zero ? r + 1 : nb - r;
+  const int64_t q = (cnt + 3) / 4;
+  const int64_t c0 = lo + g * q, c1 = lo + (g * q + q < cnt ? g * q + q : cnt);
+  E acc = ezero<E>();
+  const E *base = Q + r * nb * NB + lane;
+  int64_t c = c0;
+  for (; c + 8 <= c1; c += 8) {
+    E v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = base[(c + u) * NB];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = eadd(acc, v[u]);
+  }
+  for (; c < c1; ++c) acc = eadd(acc, base[c * NB]);
+  part[g][lane] = acc;
+  __syncthreads();
+  if (g == 0 && i < n) x[i * incx] = eadd(eadd(part[0][lane], part[1][lane]), eadd(part[2][lane], part[3][lane]));
+}
+
+// CORAL argument order: uplo=1 trans=2 diag=3 n=4 a=5 lda=6 x=7 incx=8
+template <typename E>
+static int trmv_impl(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const E *a, int64_t lda, E *x,
+                     int64_t incx, coral_stream_t stream) {
+  if (!valid_uplo(uplo)) return -1;
+  if (!valid_op(trans)) return -2;
+  if (!valid_diag(diag)) return -3;
+  if (n < 0) return -4;
+  if (lda < imax64(1, n)) return -6;
+  if (incx <= 0) return -8;
+  if (n == 0) return CORAL_OK;
+  if (!a) return -5;
+  if (!x) return -7;
+  CORAL_TRY(ensure_init());
+  hipStream_t s = resolve(stream);
+  const int64_t nb = cdiv(n, NB);
+  void *Q = nullptr;
+  CORAL_TRY(workspace(s, WS_AUX2, sizeof(E) * (size_t)(nb * nb * NB), &Q));
+  // real routines map ConjugateTranspose to Transpose (coral-aarch64/src/level2/dtrmv.rs:58-79)
+  const int op = (trans == CORAL_CONJUGATE_TRANSPOSE && !elem_traits<E>::cplx) ? 1 : trans;
+  const int upper = uplo == CORAL_UPPER_TRIANGULAR, unit = diag == CORAL_UNIT_DIAGONAL;
+  const int vec_ok = ((uintptr_t)a % 16 == 0) && ((lda * (int64_t)sizeof(E)) % 16 == 0);
+  const dim3 grid((unsigned)(nb * (nb + 1) / 2));
+  if (op == 0)
+    hipLaunchKernelGGL((trmv_block_kernel<E, 0>), grid, dim3(256), 0, s, n, upper, unit, a, lda, (const E *)x, incx,
+                       (E *)Q, nb, vec_ok);
+  else if (op == 1)
+    hipLaunchKernelGGL((trmv_block_kernel<E, 1>), grid, dim3(256), 0, s, n, upper, unit, a, lda, (const E *)x, incx,
+                       (E *)Q, nb, vec_ok);
+  else
+    hipLaunchKernelGGL((trmv_block_kernel<E, 2>), grid, dim3(256), 0, s, n, upper, unit, a, lda, (const E *)x, incx,
+                       (E *)Q, nb, vec_ok);
+  CORAL_LAUNCH_CHECK();
+  // partners on the diagonal's near side: N lower and T/C upper sum c = 0..r
+  const int from_zero = (!upper) == (op == 0);
+  hipLaunchKernelGGL(trmv_reduce_kernel<E>, dim3((unsigned)nb), dim3(256), 0, s, n, nb, (const E *)Q, from_zero, x,
+                     incx);
+  CORAL_LAUNCH_CHECK();
+  return CORAL_OK;
+}
+
 }  // namespace sym
 }  // namespace coral
 
@@ -255,5 +397,15 @@ int coral_zhemv(int32_t uplo, int64_t n, const double alpha[2], const double *a,
   return symv_impl<double2, true>(uplo, n, make_double2(alpha[0], alpha[1]), (const double2 *)a, lda,
                                   (const double2 *)x, incx, make_double2(beta[0], beta[1]), (double2 *)y, incy, s);
 }
+
+#define TRMV_ENTRY(NAME, E, T)                                                                               \
+  int coral_##NAME(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const T *a, int64_t lda, T *x,     \
+                   int64_t incx, coral_stream_t s) {                                                         \
+    return trmv_impl<E>(uplo, trans, diag, n, (const E *)a, lda, (E *)x, incx, s);                          \
+  }
+TRMV_ENTRY(strmv, float, float)
+TRMV_ENTRY(dtrmv, double, double)
+TRMV_ENTRY(ctrmv, float2, float)
+TRMV_ENTRY(ztrmv, double2, double)
 
 }  // extern "C"

@@ -1,4 +1,4 @@
-// Level-2 TRSV / TRMV for gfx950, plus batched DTRSV.
+// Level-2 TRSV for gfx950, plus batched DTRSV.
 //
 // CORAL solves op(A) x = b in NB = 8 diagonal blocks: a scalar substitution on the
 // diagonal block (coral-aarch64/src/level2/dtrlsv.rs:65-139) followed by a fused
@@ -18,8 +18,6 @@
 //   * the diagonal block is solved by one wavefront (lane = row), in the
 //     element precision (FP64 for D/Z), and published through a sentinel-filled
 //     staging vector whose words are their own ready flags (see Stage below).
-// TRMV (x := op(A) x) uses the same tiles without dependencies: one workgroup per
-// (row block, column block) pair writes a partial, a second kernel sums them.
 #include "coral_common.hpp"
 
 #include <algorithm>
@@ -344,60 +342,6 @@ __global__ void zero_ints_kernel(int *p, int count) {
   for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < count; i += blockDim.x * gridDim.x) p[i] = 0;
 }
 
-// ---- TRMV: partial y_b += M(b, j) x_j for j <= b ----
-__device__ __forceinline__ void pair_of(int64_t t, int64_t &b, int64_t &j) {
-  int64_t c = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while (c * (c + 1) / 2 > t) --c;
-  while ((c + 1) * (c + 2) / 2 <= t) ++c;
-  b = c;
-  j = t - c * (c + 1) / 2;
-}
-
-template <typename E>
-__global__ __launch_bounds__(256) void trmv_block_kernel(TriView<E> v, const E *x, int64_t incx, int unit, E *P,
-                                                         int64_t nb) {
-  __shared__ E T[NB][NB + 1];
-  __shared__ E xs[NB];
-  __shared__ E part[4][NB];
-  int64_t b, j;
-  pair_of(blockIdx.x, b, j);
-  const int t = threadIdx.x, lane = t & 63, q = t >> 6;
-  const int64_t i0 = b * NB, c0 = j * NB;
-  const bool diag = b == j;
-  E reg[16];
-  load_tile(v, i0, c0, diag, reg);
-  store_tile(v, T, reg);
-  if (t < NB) {
-    const int64_t c = c0 + t;
-    xs[t] = c < v.n ? x[v.map(c) * incx] : ezero<E>();
-  }
-  __syncthreads();
-  E acc = ezero<E>();
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const int c = q * 16 + k;
-    acc = emad(acc, T[c][lane], xs[c]);
-  }
-  part[q][lane] = acc;
-  __syncthreads();
-  if (q == 0) {
-    E s = eadd(eadd(part[0][lane], part[1][lane]), eadd(part[2][lane], part[3][lane]));
-    const int64_t i = i0 + lane;
-    if (diag && i < v.n) s = unit ? eadd(s, xs[lane]) : emad(s, v.get(i, i), xs[lane]);
-    P[(b * nb + j) * NB + lane] = s;
-  }
-}
-
-template <typename E>
-__global__ void trmv_reduce_kernel(TriView<E> v, E *x, int64_t incx, const E *P, int64_t nb) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= v.n) return;
-  const int64_t b = i / NB, w = i % NB;
-  E s = ezero<E>();
-  for (int64_t j = 0; j <= b; ++j) s = eadd(s, P[(b * nb + j) * NB + w]);
-  x[v.map(i) * incx] = s;
-}
-
 // CORAL argument order: uplo=1 trans=2 diag=3 n=4 a=5 lda=6 x=7 incx=8
 template <typename E>
 static int check_tr(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const E *a, int64_t lda, E *x,
@@ -450,26 +394,6 @@ static int trsv_impl(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const
   else
     hipLaunchKernelGGL((trsv_kernel<E, 0>), dim3((unsigned)nb), dim3(256), 0, s, v, x, incx,
                        (int)(diag == CORAL_UNIT_DIAGONAL), (W *)stage, (int *)ctrl, nb);
-  CORAL_LAUNCH_CHECK();
-  return CORAL_OK;
-}
-
-template <typename E>
-static int trmv_impl(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const E *a, int64_t lda, E *x,
-                     int64_t incx, coral_stream_t stream) {
-  CORAL_TRY(check_tr(uplo, trans, diag, n, a, lda, x, incx));
-  if (n == 0) return CORAL_OK;
-  CORAL_TRY(ensure_init());
-  hipStream_t s = resolve(stream);
-  const int64_t nb = cdiv(n, NB);
-  void *P = nullptr;
-  CORAL_TRY(workspace(s, WS_AUX2, sizeof(E) * (size_t)(nb * nb * NB), &P));
-  TriView<E> v = make_view(uplo, trans, a, lda, n);
-  hipLaunchKernelGGL(trmv_block_kernel<E>, dim3((unsigned)(nb * (nb + 1) / 2)), dim3(256), 0, s, v, (const E *)x,
-                     incx, (int)(diag == CORAL_UNIT_DIAGONAL), (E *)P, nb);
-  CORAL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(trmv_reduce_kernel<E>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, v, x, incx,
-                     (const E *)P, nb);
   CORAL_LAUNCH_CHECK();
   return CORAL_OK;
 }
@@ -558,16 +482,6 @@ TRSV_ENTRY(strsv, float, float)
 TRSV_ENTRY(dtrsv, double, double)
 TRSV_ENTRY(ctrsv, float2, float)
 TRSV_ENTRY(ztrsv, double2, double)
-
-#define TRMV_ENTRY(NAME, E, T)                                                                               \
-  int coral_##NAME(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const T *a, int64_t lda, T *x,     \
-                   int64_t incx, coral_stream_t s) {                                                         \
-    return trmv_impl<E>(uplo, trans, diag, n, (const E *)a, lda, (E *)x, incx, s);                          \
-  }
-TRMV_ENTRY(strmv, float, float)
-TRMV_ENTRY(dtrmv, double, double)
-TRMV_ENTRY(ctrmv, float2, float)
-TRMV_ENTRY(ztrmv, double2, double)
 
 // uplo=1 trans=2 diag=3 batch=4 n=5 a=6 lda=7 x=8
 int coral_dtrsv_batched(int32_t uplo, int32_t trans, int32_t diag, int64_t batch, const int32_t *n,

@@ -140,6 +140,49 @@ def test_trsv_trmv_all_variants(gpu_lib, p):
                     assert allclose_ref(xs, x, 1e-4 if p in "sc" else 1e-12, 1e-5 if p in "sc" else 1e-12)
 
 
+def _tri_fast(n, lda, f, rng, uplo, cplx):
+    """Stored triangle of U[-1,1] values (complex: both parts), NaN everywhere else
+    in the lda x n buffer, so any read outside the triangle shows up."""
+    a = np.full((n, lda), np.nan, f)  # a[j, i] = A(i, j)
+    v = rand(rng, n * n, f).reshape(n, n)
+    keep = np.tril(np.ones((n, n), bool)) if uplo == 0 else np.triu(np.ones((n, n), bool))
+    a[:, :n] = np.where(keep, v, np.nan)  # keep[j, i]: row i of column j is stored
+    return a.reshape(-1)
+
+
+@pytest.mark.parametrize("p", "sdcz")
+def test_trmv_against_fp64(gpu_lib, p):
+    """x := op(A) x for every uplo x trans x diag, ragged n around the 64-row blocks,
+    padded lda and strided x: elementwise within 2*n*eps*(|op(A)| |x|) of FP64, with
+    NaN in the unstored triangle, the padding and (unit) the diagonal."""
+    cplx = p in "cz"
+    f = DT[p]
+    rng = np.random.default_rng(24)
+    eps = 2.0 ** -24 if p in "sc" else 2.0 ** -53
+    for n, lda, incx in ((1, 1, 1), (63, 63, 1), (64, 66, 2), (129, 129, 1), (1000, 1003, 3), (1537, 1537, 1)):
+        for uplo in (0, 1):
+            a = _tri_fast(n, lda, f, rng, uplo, cplx)
+            for trans in (0, 1, 2):
+                for diag in (0, 1):
+                    a_use = a.copy()
+                    if diag == 0:
+                        a_use.reshape(n, lda)[np.arange(n), np.arange(n)] = np.nan  # never read
+                    x = rand(rng, (n - 1) * incx + 1, f)
+                    xs = x.copy()
+                    getattr(blas, f"{p}trmv")(uplo, trans, diag, n, a_use, lda, xs, incx)
+                    ref = ref64.trmv(uplo, trans, diag, n, a_use, lda, x, incx)
+                    M = ref64.triangular(a_use, n, lda, uplo, trans, diag)
+                    scale = np.abs(M) @ np.abs(ref64.vec(x, n, incx).astype(M.dtype))
+                    got = ref64.vec(xs, n, incx).astype(M.dtype)
+                    want = ref64.vec(ref, n, incx)
+                    err = np.abs(got - want)
+                    assert np.all(err <= 2 * n * eps * scale + 1e-300), (n, uplo, trans, diag, float(np.max(err / scale)))
+                    # elements between the strided entries are untouched
+                    mask = np.ones(xs.shape[0], bool)
+                    mask[::incx] = False
+                    assert np.array_equal(xs[mask], x[mask])
+
+
 def test_dtrsv_n8192_backward_error(gpu_lib):
     # the DTRSV configuration: lower, non-unit, n = 8192, strictly-lower U[-1,1]/n, diag U[1,2]
     rng = np.random.default_rng(23)
