@@ -40,24 +40,31 @@ struct TriView {
     E v = A[rc + ri * lda];
     return trans == 2 ? econj(v) : v;
   }
-  // M(i, c) when the layout is known at compile time (RC: trans == 0)
+  // The stored element behind M(i, c) when the layout is known at compile time
+  // (RC: trans == 0).  For trans != 0 this is A's element before conjugation: a
+  // select on the value right after its load would make every prefetched tile
+  // load wait for completion, so ConjTrans conjugates later, in store_tile.
   template <bool RC>
-  __device__ __forceinline__ E get_t(int64_t i, int64_t c) const {
+  __device__ __forceinline__ E get_raw(int64_t i, int64_t c) const {
     const int64_t ri = map(i), rc = map(c);
-    if constexpr (RC) {
-      return A[ri + rc * lda];
-    } else {
-      E v = A[rc + ri * lda];
-      return trans == 2 ? econj(v) : v;
-    }
+    if constexpr (RC) return A[ri + rc * lda];
+    else return A[rc + ri * lda];
   }
   // true when consecutive i are adjacent in memory (coalesced thread-per-row loads)
   __device__ __forceinline__ bool row_contig() const { return trans == 0; }
 };
 
 // Loads tile (rows i0.., cols c0..) of M into registers, 16 elements per thread.
-// diag: only the strictly lower part (c < i) is read; the rest is zero.
+// diag: only the strictly lower part (c < i) counts; the rest is zero.
 // RCT: -1 decides the layout at run time; 0 / 1 fix it at compile time (1: trans == 0).
+//
+// With RCT >= 0 every load is unconditional, from an index clamped into the n x n
+// matrix, and the registers hold A's elements as stored; store_tile applies the
+// mask (rows/columns past n, the upper part of a diagonal tile) and ConjTrans's
+// conjugate when it writes the tile to LDS, so the two are always used as a pair.
+// Any use of a loaded value here (a select, a conjugate, a guarded load merged
+// into another register) makes the compiler wait for that load at once, which
+// would put a full memory round trip on every prefetched tile.
 template <int RCT = -1, typename E>
 __device__ __forceinline__ void load_tile(const TriView<E> &v, int64_t i0, int64_t c0, bool diag, E (&reg)[16]) {
   const int t = threadIdx.x;
@@ -68,21 +75,33 @@ __device__ __forceinline__ void load_tile(const TriView<E> &v, int64_t i0, int64
     const int r = rc ? (t & 63) : ((t >> 6) + 4 * k);
     const int c = rc ? ((t >> 6) + 4 * k) : (t & 63);
     const int64_t i = i0 + r, cc = c0 + c;
-    const bool ok = i < v.n && cc < v.n && (!diag || c < r);
-    if constexpr (RCT < 0) reg[k] = ok ? v.get(i, cc) : ezero<E>();
-    else reg[k] = ok ? v.template get_t<RCT == 1>(i, cc) : ezero<E>();
+    if constexpr (RCT < 0) {
+      const bool ok = i < v.n && cc < v.n && (!diag || c < r);
+      reg[k] = ok ? v.get(i, cc) : ezero<E>();
+    } else {
+      reg[k] = v.template get_raw<RCT == 1>(i < v.n ? i : v.n - 1, cc < v.n ? cc : v.n - 1);
+    }
   }
 }
 
 template <int RCT = -1, typename E>
-__device__ __forceinline__ void store_tile(const TriView<E> &v, E (*T)[NB + 1], const E (&reg)[16]) {
+__device__ __forceinline__ void store_tile(const TriView<E> &v, E (*T)[NB + 1], const E (&reg)[16], int64_t i0,
+                                           int64_t c0, bool diag) {
   const int t = threadIdx.x;
   const bool rc = RCT < 0 ? v.row_contig() : (RCT == 1);
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const int r = rc ? (t & 63) : ((t >> 6) + 4 * k);
     const int c = rc ? ((t >> 6) + 4 * k) : (t & 63);
-    T[c][r] = reg[k];  // column-major tile: T[c][r] = M(i0 + r, c0 + c)
+    // column-major tile: T[c][r] = M(i0 + r, c0 + c)
+    if constexpr (RCT < 0) {
+      T[c][r] = reg[k];
+    } else {
+      const bool ok = i0 + r < v.n && c0 + c < v.n && (!diag || c < r);
+      E e = reg[k];
+      if constexpr (RCT == 0) e = v.trans == 2 ? econj(e) : e;
+      T[c][r] = ok ? e : ezero<E>();
+    }
   }
 }
 
@@ -233,7 +252,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
   {
     E dreg[16];
     load_tile<RC>(v, i0, i0, true, dreg);
-    store_tile<RC>(v, Td, dreg);
+    store_tile<RC>(v, Td, dreg, i0, i0, true);
     __syncthreads();
     const int64_t i = i0 + lane;
     const E d = (unit || i >= v.n) ? eone<E>() : v.get(i, i);
@@ -277,7 +296,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
   E reg0[16], reg1[16];
   auto step = [&](int j, E(&cur)[16]) {
     __syncthreads();  // every wavefront is done reading T and xs of tile j-1
-    store_tile<RC>(v, T, cur);
+    store_tile<RC>(v, T, cur, i0, (int64_t)j * NB, false);
     if (q == 0) {
       E val = xnext;
       if (xnext_j != j) {
@@ -434,7 +453,7 @@ __global__ __launch_bounds__(256) void trsv_batched_kernel(int uplo, int trans, 
       double acc = 0.0;
       for (int j = 0; j < b; ++j) {
         load_tile(v, i0, (int64_t)j * NB, false, reg);
-        store_tile(v, T, reg);
+        store_tile(v, T, reg, i0, (int64_t)j * NB, false);
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
@@ -445,7 +464,7 @@ __global__ __launch_bounds__(256) void trsv_batched_kernel(int uplo, int trans, 
       }
       part[q][lane] = acc;
       load_tile(v, i0, i0, true, reg);
-      store_tile(v, T, reg);
+      store_tile(v, T, reg, i0, i0, true);
       __syncthreads();
       if (q == 0) {
         const int64_t i = i0 + lane;
