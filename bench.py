@@ -616,7 +616,9 @@ def suite_rows(parts=("gemv", "symv", "tr", "rank", "l1")):
                     extra = {"backward_error": float(np.linalg.norm(bb - M @ xs) /
                                                      (np.linalg.norm(M) * np.linalg.norm(xs) + np.linalg.norm(bb)))}
                 cpu = None
-                if p == "d":
+                # the oracle's TRMV is a plain row-order restatement (a checker, not
+                # CORAL's blocked dtrlmv), so only TRSV gets a CPU leg
+                if p == "d" and routine == "trsv":
                     xc = b.copy()
                     cpu = cpu_leg(lambda: (xc.__setitem__(slice(None), b),
                                            getattr(port, f"d{routine}")(1, trans, 1, n, af, n, xc, 1)),
