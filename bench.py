@@ -24,7 +24,8 @@ settles over the first tens of milliseconds of sustained MFMA load.
 
 `cpu_baseline` times the oracle's restatement of CORAL's CPU SGEMM (the Goto
 5-loop with CORAL's blocking, oracle/coral_port.c) on one host core on a bounded
-sample (SGEMM NN 3072^3, median of 3 runs).
+sample (SGEMM NN 3584^3, one warm-up run and the median of 5 runs: about 12 s of
+CPU work).
 """
 from __future__ import annotations
 
@@ -141,7 +142,7 @@ def cpu_leg(fn, work, unit, scale, sample, runs=5):
                       f"{os.cpu_count()} ({_cpu_model()}), median of {runs} runs after 1 warm-up, {t:.3f} s per run"}
 
 
-def cpu_baseline(n=3072, runs=5):
+def cpu_baseline(n=3584, runs=5):
     """CORAL's CPU SGEMM algorithm (oracle restatement) on one core."""
     from oracle import port
 
