@@ -23,6 +23,11 @@ typedef uint32_t l1v4u __attribute__((ext_vector_type(4)));
 // nontemporal loads, kU vectors in flight per thread, each wave instruction covering
 // one contiguous 1 KiB run; other inputs one element per thread per step.  Each
 // thread visits its elements in increasing index order.  f(i, x_i, y_i).
+//
+// The vector loops run every step that lies wholly inside the vectors without
+// per-vector guards: a guarded load is a branch, and at each branch join the
+// compiler waits for every load issued so far, which left one load in flight per
+// thread instead of kU.  Only a workgroup's last, partial step is guarded.
 constexpr int kU = 4;
 template <typename E, bool Y, class F>
 __device__ __forceinline__ void stream_visit(int64_t n, const E *__restrict__ x, int64_t incx,
@@ -34,12 +39,13 @@ __device__ __forceinline__ void stream_visit(int64_t n, const E *__restrict__ x,
     const int64_t nv = n / EV;  // whole vectors
     const l1v4u *xv = reinterpret_cast<const l1v4u *>(x);
     const l1v4u *yv = reinterpret_cast<const l1v4u *>(y);
-    for (int64_t base = (int64_t)blockIdx.x * kThreads * kU; base < nv; base += nt * kU) {
+    auto step = [&](int64_t base, auto guarded) {
+      constexpr bool G = decltype(guarded)::value;
       l1v4u a[kU], b[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
         const int64_t v = base + u * kThreads + t;
-        if (v < nv) {
+        if (!G || v < nv) {
           a[u] = __builtin_nontemporal_load(xv + v);
           if constexpr (Y) b[u] = __builtin_nontemporal_load(yv + v);
         }
@@ -47,7 +53,7 @@ __device__ __forceinline__ void stream_visit(int64_t n, const E *__restrict__ x,
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
         const int64_t v = base + u * kThreads + t;
-        if (v < nv) {
+        if (!G || v < nv) {
           E ea[EV], eb[EV];
           __builtin_memcpy(ea, &a[u], 16);
           if constexpr (Y) __builtin_memcpy(eb, &b[u], 16);
@@ -55,6 +61,10 @@ __device__ __forceinline__ void stream_visit(int64_t n, const E *__restrict__ x,
           for (int e = 0; e < EV; ++e) f(v * EV + e, ea[e], Y ? eb[e] : ea[e]);
         }
       }
+    };
+    for (int64_t base = (int64_t)blockIdx.x * kThreads * kU; base < nv; base += nt * kU) {
+      if (base + kU * kThreads <= nv) step(base, std::false_type{});  // whole step in range: no guards
+      else step(base, std::true_type{});
     }
     done = nv * EV;
   }
@@ -160,17 +170,18 @@ __global__ __launch_bounds__(kThreads) void axpy_vec_kernel(int64_t n, int64_t n
   const int64_t nv = n / EV, nt = (int64_t)gridDim.x * kThreads;
   const l1v4u *xv = reinterpret_cast<const l1v4u *>(x);
   l1v4u *yv = reinterpret_cast<l1v4u *>(y);
-  for (int64_t base = (int64_t)blockIdx.x * kThreads * kU; base < nv; base += nt * kU) {
+  auto step = [&](int64_t base, auto guarded) {  // guards only on a partial step (see stream_visit)
+    constexpr bool G = decltype(guarded)::value;
     l1v4u a[kU], b[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int64_t v = base + u * kThreads + threadIdx.x;
-      if (v < nv) { a[u] = __builtin_nontemporal_load(xv + v); b[u] = yv[v]; }
+      if (!G || v < nv) { a[u] = __builtin_nontemporal_load(xv + v); b[u] = yv[v]; }
     }
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int64_t v = base + u * kThreads + threadIdx.x;
-      if (v < nv) {
+      if (!G || v < nv) {
         E ea[EV], eb[EV];
         __builtin_memcpy(ea, &a[u], 16);
         __builtin_memcpy(eb, &b[u], 16);
@@ -180,6 +191,10 @@ __global__ __launch_bounds__(kThreads) void axpy_vec_kernel(int64_t n, int64_t n
         yv[v] = b[u];
       }
     }
+  };
+  for (int64_t base = (int64_t)blockIdx.x * kThreads * kU; base < nv; base += nt * kU) {
+    if (base + kU * kThreads <= nv) step(base, std::false_type{});
+    else step(base, std::true_type{});
   }
   for (int64_t i = nv * EV + (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += nt)
     y[i] = axpy_upd(y[i], alpha, x[i], i < nf);
@@ -252,17 +267,18 @@ __global__ __launch_bounds__(kThreads) void scal_vec_kernel(int64_t n, int64_t n
   constexpr int EV = 16 / sizeof(E);
   const int64_t nv = n / EV, nt = (int64_t)gridDim.x * kThreads;
   l1v4u *xv = reinterpret_cast<l1v4u *>(x);
-  for (int64_t base = (int64_t)blockIdx.x * kThreads * kU; base < nv; base += nt * kU) {
+  auto step = [&](int64_t base, auto guarded) {  // guards only on a partial step (see stream_visit)
+    constexpr bool G = decltype(guarded)::value;
     l1v4u a[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int64_t v = base + u * kThreads + threadIdx.x;
-      if (v < nv) a[u] = __builtin_nontemporal_load(xv + v);
+      if (!G || v < nv) a[u] = __builtin_nontemporal_load(xv + v);
     }
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int64_t v = base + u * kThreads + threadIdx.x;
-      if (v < nv) {
+      if (!G || v < nv) {
         E ea[EV];
         __builtin_memcpy(ea, &a[u], 16);
 #pragma unroll
@@ -271,6 +287,10 @@ __global__ __launch_bounds__(kThreads) void scal_vec_kernel(int64_t n, int64_t n
         __builtin_nontemporal_store(a[u], xv + v);
       }
     }
+  };
+  for (int64_t base = (int64_t)blockIdx.x * kThreads * kU; base < nv; base += nt * kU) {
+    if (base + kU * kThreads <= nv) step(base, std::false_type{});
+    else step(base, std::true_type{});
   }
   for (int64_t i = nv * EV + (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += nt)
     x[i] = scal_upd(x[i], alpha, i < nf);
@@ -550,12 +570,13 @@ __global__ __launch_bounds__(kThreads) void xy_map_kernel(int64_t n, E *x, int64
     const int64_t nv = n / EV;
     l1v4u *xv = reinterpret_cast<l1v4u *>(x);
     l1v4u *yv = reinterpret_cast<l1v4u *>(y);
-    for (int64_t base = (int64_t)blockIdx.x * kThreads * kU; base < nv; base += nt * kU) {
+    auto step = [&](int64_t base, auto guarded) {  // guards only on a partial step (see stream_visit)
+      constexpr bool G = decltype(guarded)::value;
       l1v4u a[kU], b[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
         const int64_t v = base + u * kThreads + threadIdx.x;
-        if (v < nv) {
+        if (!G || v < nv) {
           a[u] = __builtin_nontemporal_load(xv + v);
           if (WX) b[u] = __builtin_nontemporal_load(yv + v);
         }
@@ -563,7 +584,7 @@ __global__ __launch_bounds__(kThreads) void xy_map_kernel(int64_t n, E *x, int64
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
         const int64_t v = base + u * kThreads + threadIdx.x;
-        if (v < nv) {
+        if (!G || v < nv) {
           E ea[EV], eb[EV];
           __builtin_memcpy(ea, &a[u], 16);
           if (WX) __builtin_memcpy(eb, &b[u], 16);
@@ -577,6 +598,10 @@ __global__ __launch_bounds__(kThreads) void xy_map_kernel(int64_t n, E *x, int64
           __builtin_nontemporal_store(b[u], yv + v);
         }
       }
+    };
+    for (int64_t base = (int64_t)blockIdx.x * kThreads * kU; base < nv; base += nt * kU) {
+      if (base + kU * kThreads <= nv) step(base, std::false_type{});
+      else step(base, std::true_type{});
     }
     done = nv * EV;
   }
