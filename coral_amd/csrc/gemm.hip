@@ -183,6 +183,17 @@ __device__ __forceinline__ void load_run(const RealJob<T> &j, int64_t r, int64_t
   }
 }
 
+// The 16-byte nontemporal load of load_run without its guards, for workgroups whose
+// whole block lies inside the operand.  A guarded load is a branch, and at each
+// branch join the compiler waits for every load issued so far; unguarded, all of a
+// thread's loads of the block are in flight at once.
+template <typename T, int LV>
+__device__ __forceinline__ void load_run_full(const RealJob<T> &j, int64_t r, int64_t k, T (&v)[LV]) {
+  const T *src = j.rows_contig ? j.p + r + k * j.sk : j.p + r * j.sr + k;
+  const v4u x = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(src));
+  __builtin_memcpy(v, &x, 16);
+}
+
 template <class TR, int BK>
 __global__ __launch_bounds__(256) void pack_real_kernel(const RealJob<typename TR::T> ja,
                                                         const RealJob<typename TR::T> jb) {
@@ -208,25 +219,37 @@ __global__ __launch_bounds__(256) void pack_real_kernel(const RealJob<typename T
     r0 = (blk % j.rblocks) * RW;
     k0 = (blk / j.rblocks) * BK;
     rspan = RW; kspan = BK; ld = RW + PADR;
+    T v[BK / 4][LV];
+    if (j.vec && r0 + RW <= j.R && k0 + BK <= j.K) {  // whole block inside: unguarded loads
+#pragma unroll
+      for (int c = 0; c < BK / 4; ++c) load_run_full<T, LV>(j, r0 + LV * (t & 63), k0 + (t >> 6) + 4 * c, v[c]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < BK / 4; ++c) load_run<T, LV>(j, r0 + LV * (t & 63), k0 + (t >> 6) + 4 * c, v[c]);
+    }
 #pragma unroll
     for (int c = 0; c < BK / 4; ++c) {
       const int kl = (t >> 6) + 4 * c, rl = LV * (t & 63);
-      T v[LV];
-      load_run<T, LV>(j, r0 + rl, k0 + kl, v);
 #pragma unroll
-      for (int e = 0; e < LV; ++e) s[kl * ld + rl + e] = v[e];  // s[k][r]
+      for (int e = 0; e < LV; ++e) s[kl * ld + rl + e] = v[c][e];  // s[k][r]
     }
   } else {
     k0 = (blk % j.kblocks) * KW;
     r0 = (blk / j.kblocks) * 64;
     rspan = 64; kspan = KW; ld = KW + LV;
+    T v[8][LV];
+    if (j.vec && r0 + 64 <= j.R && k0 + KW <= j.K) {  // whole block inside: unguarded loads
+#pragma unroll
+      for (int c = 0; c < 8; ++c) load_run_full<T, LV>(j, r0 + (t >> 5) + 8 * c, k0 + LV * (t & 31), v[c]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) load_run<T, LV>(j, r0 + (t >> 5) + 8 * c, k0 + LV * (t & 31), v[c]);
+    }
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const int rl = (t >> 5) + 8 * c, kl = LV * (t & 31);
-      T v[LV];
-      load_run<T, LV>(j, r0 + rl, k0 + kl, v);
       v4u x;
-      __builtin_memcpy(&x, v, 16);
+      __builtin_memcpy(&x, v[c], 16);
       *reinterpret_cast<v4u *>(&s[rl * ld + kl]) = x;  // s[r][k], 16-byte aligned rows
     }
   }
@@ -288,12 +311,22 @@ __global__ __launch_bounds__(256) void absmax_real_kernel(const RealJob<float> j
     const int64_t r0 = (blk % j.rblocks) * 256, k0 = (blk / j.rblocks) * KCH_ROWS;
     const int64_t r = r0 + 4 * lane;
     float m[4] = {0.f, 0.f, 0.f, 0.f};
+    if (j.vec && r0 + 256 <= j.R && k0 + KCH_ROWS <= j.K) {  // whole block inside: unguarded loads
 #pragma unroll 8
-    for (int c = 0; c < KCH_ROWS / 4; ++c) {
-      float v[4];
-      load_run<float, 4>(j, r, k0 + w + 4 * c, v);
+      for (int c = 0; c < KCH_ROWS / 4; ++c) {
+        float v[4];
+        load_run_full<float, 4>(j, r, k0 + w + 4 * c, v);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) m[e] = fmaxf(m[e], fabsf(v[e]));
+        for (int e = 0; e < 4; ++e) m[e] = fmaxf(m[e], fabsf(v[e]));
+      }
+    } else {
+#pragma unroll 8
+      for (int c = 0; c < KCH_ROWS / 4; ++c) {
+        float v[4];
+        load_run<float, 4>(j, r, k0 + w + 4 * c, v);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) m[e] = fmaxf(m[e], fabsf(v[e]));
+      }
     }
 #pragma unroll
     for (int e = 0; e < 4; ++e) red[w][4 * lane + e] = m[e];
@@ -303,7 +336,14 @@ __global__ __launch_bounds__(256) void absmax_real_kernel(const RealJob<float> j
   } else {
     const int64_t r = (blk % j.rblocks) * 4 + w, k0 = (blk / j.rblocks) * KCH_K;
     float m = 0.f;
-    if (r < j.R) {
+    if (j.vec && r < j.R && k0 + KCH_K <= j.K) {  // this wave's whole run inside: unguarded loads
+#pragma unroll 8
+      for (int c = 0; c < KCH_K / 256; ++c) {
+        float v[4];
+        load_run_full<float, 4>(j, r, k0 + 256 * c + 4 * lane, v);
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))));
+      }
+    } else if (r < j.R) {
 #pragma unroll 4
       for (int c = 0; c < KCH_K / 256; ++c) {
         float v[4];
