@@ -24,6 +24,7 @@
 #include "coral_common.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace coral {
 namespace rk {
@@ -138,6 +139,18 @@ __device__ __forceinline__ void load_src(E (&v)[16 / sizeof(E)], const E *s, int
   }
 }
 
+// v as held by lane l of the wavefront (l wave-uniform), word by word.
+template <typename E>
+__device__ __forceinline__ E wave_bcast(E v, int l) {
+  constexpr int W = sizeof(E) / 4;
+  uint32_t u[W];
+  __builtin_memcpy(u, &v, sizeof(E));
+#pragma unroll
+  for (int k = 0; k < W; ++k) u[k] = (uint32_t)__builtin_amdgcn_readlane((int)u[k], l);
+  __builtin_memcpy(&v, u, sizeof(E));
+  return v;
+}
+
 // Largest g with g(g+1)/2 <= q.
 __device__ __forceinline__ int64_t tri_root(int64_t q) {
   int64_t g = (int64_t)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
@@ -186,7 +199,7 @@ __global__ __launch_bounds__(kThreads) void rank_kernel(RankArgs<E> p) {
     cb = KT * grp + w0 / (g + 1);
     if (cb * p.cpc >= p.n || rb * RB >= p.m) return;
   }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t r0 = rb * RB;
   const int64_t c0 = cb * p.cpc;
   const int64_t c1 = std::min<int64_t>(c0 + p.cpc, p.n);
@@ -198,71 +211,143 @@ __global__ __launch_bounds__(kThreads) void rank_kernel(RankArgs<E> p) {
   load_src<E>(xs, p.x, p.incx, p.vec_x, rr, p.m);
   if (TWO) load_src<E>(ys, p.y, p.incy, p.vec_y, rr, p.m);
 
-  for (int64_t j0 = c0; j0 < c1; j0 += kU) {
-    E av[kU][EV];
-    E cf1[kU], cf2[kU];
-    // loads of kU columns first (coefficients and flags are wave-uniform)
+  // Columns in groups of 64: lane t computes the coefficients of column g0 + t once,
+  // and the column loop takes them from that lane.  (Loaded per column inside the
+  // loop, each coefficient is a load that the next update waits for, together with
+  // every load of A issued before it.)  Real double is the exception: its columns
+  // stream fastest with the coefficient loaded beside each column and one column in
+  // flight per wave.  Measured at 16384^2 / 8192^2, two runs each: DGER 6.02-6.04
+  // TB/s that way, 5.30-5.41 with broadcast coefficients and the unguarded path
+  // below, 5.56-5.83 with broadcast coefficients alone; the other element types gain
+  // from both (SGER 5.60 -> 6.26-6.28, CGERU 5.53-5.57 -> 6.15-6.17, ZGERC
+  // 5.49-5.58 -> 5.78-5.81, CHER 4.90-4.93 -> 5.01-5.02, ZHER2 4.55-4.60 -> 5.21-5.24).
+  constexpr bool kWide = !std::is_same<E, double>::value;
+  for (int64_t g0 = c0; g0 < c1; g0 += 64) {
+    const int64_t g1 = std::min<int64_t>(g0 + 64, c1);
+    E lc1 = ezero<E>(), lc2 = ezero<E>();
+    if (kWide && g0 + lane < g1) column_coefs<E, K>(p, g0 + lane, lc1, lc2);
+    for (int64_t j0 = g0; j0 < g1; j0 += kU) {
+      E av[kU][EV];
+      E cf1[kU], cf2[kU];
+      // Fast path (wave-uniform): all kU columns exist and this wave's rows lie inside
+      // every one's stored rows.  Loads and stores are then unconditional: a guarded
+      // load is a branch, and at each branch join the compiler waits for every load
+      // issued so far, which would leave one column in flight instead of kU.  A
+      // column CORAL skips (zero coefficient) is read and written back unchanged
+      // here, which leaves the same bits.
+      bool fast = kWide && p.vec_a && j0 + kU <= g1;
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int64_t j = j0 + u;
-      if (j >= c1) continue;
-      column_coefs<E, K>(p, j, cf1[u], cf2[u]);
-      bool a1, a2, zd;
-      column_flags<E, K>(p, cf1[u], cf2[u], a1, a2, zd);
-      int64_t l, h;
-      column_rows<K>(p.m, p.n, p.lower, j, l, h);
-      if (!(a1 || a2)) {  // at most the diagonal changes (HER2 with zero coefficients)
-        if (!zd) continue;
-        l = j;
-        h = j + 1;
+      for (int u = 0; u < kU; ++u) {
+        int64_t l, h;
+        column_rows<K>(p.m, p.n, p.lower, j0 + u, l, h);
+        fast = fast && rw >= l && rw + 64 * EV <= h;
       }
-      if (rw >= h || rw + 64 * EV <= l) continue;  // no row of this wave
-      const E *col = p.a + j * p.lda;
-      if (p.vec_a && rr >= l && rr + EV <= h) {
-        const rv4u pk = __builtin_nontemporal_load(reinterpret_cast<const rv4u *>(col + rr));
-        __builtin_memcpy(&av[u][0], &pk, 16);
-      } else {
+      if (fast) {
 #pragma unroll
-        for (int e = 0; e < EV; ++e) av[u][e] = (rr + e >= l && rr + e < h) ? col[rr + e] : ezero<E>();
-      }
-    }
-    // then the updates and stores; the per-column state is recomputed from j
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int64_t j = j0 + u;
-      if (j >= c1) continue;
-      bool a1, a2, zd;
-      column_flags<E, K>(p, cf1[u], cf2[u], a1, a2, zd);
-      int64_t l, h;
-      column_rows<K>(p.m, p.n, p.lower, j, l, h);
-      const int64_t seg0 = l;
-      const int64_t nf1 = fused_prefix<E>(p.mode1, h - l), nf2 = fused_prefix<E>(p.mode2, h - l);
-      if (!(a1 || a2)) {
-        if (!zd) continue;
-        l = j;
-        h = j + 1;
-      }
-      if (rw >= h || rw + 64 * EV <= l) continue;
-#pragma unroll
-      for (int e = 0; e < EV; ++e) {
-        const int64_t i = rr + e, t = i - seg0;
-        E v = av[u][e];
-        if (a1) v = axpy_upd(v, cf1[u], xs[e], t < nf1);
-        if (TWO && a2) v = axpy_upd(v, cf2[u], ys[e], t < nf2);
-        if constexpr (elem_traits<E>::cplx) {
-          if (zd && i == j) v.y = 0;
+        for (int u = 0; u < kU; ++u) {
+          const rv4u pk = __builtin_nontemporal_load(reinterpret_cast<const rv4u *>(p.a + (j0 + u) * p.lda + rr));
+          __builtin_memcpy(&av[u][0], &pk, 16);
         }
-        av[u][e] = v;
-      }
-      E *col = p.a + j * p.lda;
-      if (p.vec_a && rr >= l && rr + EV <= h) {
-        rv4u pk;
-        __builtin_memcpy(&pk, &av[u][0], 16);
-        __builtin_nontemporal_store(pk, reinterpret_cast<rv4u *>(col + rr));
-      } else {
 #pragma unroll
-        for (int e = 0; e < EV; ++e)
-          if (rr + e >= l && rr + e < h) col[rr + e] = av[u][e];
+        for (int u = 0; u < kU; ++u) {
+          const int64_t j = j0 + u;
+          cf1[u] = wave_bcast(lc1, (int)(j - g0));
+          cf2[u] = TWO ? wave_bcast(lc2, (int)(j - g0)) : ezero<E>();
+          bool a1, a2, zd;
+          column_flags<E, K>(p, cf1[u], cf2[u], a1, a2, zd);
+          int64_t l, h;
+          column_rows<K>(p.m, p.n, p.lower, j, l, h);
+          const int64_t nf1 = fused_prefix<E>(p.mode1, h - l), nf2 = fused_prefix<E>(p.mode2, h - l);
+#pragma unroll
+          for (int e = 0; e < EV; ++e) {
+            const int64_t i = rr + e, t = i - l;
+            E v = av[u][e];
+            if (a1) v = axpy_upd(v, cf1[u], xs[e], t < nf1);
+            if (TWO && a2) v = axpy_upd(v, cf2[u], ys[e], t < nf2);
+            if constexpr (elem_traits<E>::cplx) {
+              if (zd && i == j) v.y = 0;
+            }
+            av[u][e] = v;
+          }
+        }
+        // stores after all updates: a store issued between them would be waited on
+        // at the next branch join
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          rv4u pk;
+          __builtin_memcpy(&pk, &av[u][0], 16);
+          __builtin_nontemporal_store(pk, reinterpret_cast<rv4u *>(p.a + (j0 + u) * p.lda + rr));
+        }
+        continue;
+      }
+      // loads of kU columns first (coefficients and flags are wave-uniform)
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int64_t j = j0 + u;
+        if (j >= g1) continue;
+        if constexpr (kWide) {
+          cf1[u] = wave_bcast(lc1, (int)(j - g0));
+          cf2[u] = TWO ? wave_bcast(lc2, (int)(j - g0)) : ezero<E>();
+        } else {
+          column_coefs<E, K>(p, j, cf1[u], cf2[u]);
+        }
+        bool a1, a2, zd;
+        column_flags<E, K>(p, cf1[u], cf2[u], a1, a2, zd);
+        int64_t l, h;
+        column_rows<K>(p.m, p.n, p.lower, j, l, h);
+        if (!(a1 || a2)) {  // at most the diagonal changes (HER2 with zero coefficients)
+          if (!zd) continue;
+          l = j;
+          h = j + 1;
+        }
+        if (rw >= h || rw + 64 * EV <= l) continue;  // no row of this wave
+        const E *col = p.a + j * p.lda;
+        if (p.vec_a && rr >= l && rr + EV <= h) {
+          const rv4u pk = __builtin_nontemporal_load(reinterpret_cast<const rv4u *>(col + rr));
+          __builtin_memcpy(&av[u][0], &pk, 16);
+        } else {
+#pragma unroll
+          for (int e = 0; e < EV; ++e) av[u][e] = (rr + e >= l && rr + e < h) ? col[rr + e] : ezero<E>();
+        }
+      }
+      // then the updates and stores; the per-column state is recomputed from j
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int64_t j = j0 + u;
+        if (j >= g1) continue;
+        bool a1, a2, zd;
+        column_flags<E, K>(p, cf1[u], cf2[u], a1, a2, zd);
+        int64_t l, h;
+        column_rows<K>(p.m, p.n, p.lower, j, l, h);
+        const int64_t seg0 = l;
+        const int64_t nf1 = fused_prefix<E>(p.mode1, h - l), nf2 = fused_prefix<E>(p.mode2, h - l);
+        if (!(a1 || a2)) {
+          if (!zd) continue;
+          l = j;
+          h = j + 1;
+        }
+        if (rw >= h || rw + 64 * EV <= l) continue;
+#pragma unroll
+        for (int e = 0; e < EV; ++e) {
+          const int64_t i = rr + e, t = i - seg0;
+          E v = av[u][e];
+          if (a1) v = axpy_upd(v, cf1[u], xs[e], t < nf1);
+          if (TWO && a2) v = axpy_upd(v, cf2[u], ys[e], t < nf2);
+          if constexpr (elem_traits<E>::cplx) {
+            if (zd && i == j) v.y = 0;
+          }
+          av[u][e] = v;
+        }
+        E *col = p.a + j * p.lda;
+        if (p.vec_a && rr >= l && rr + EV <= h) {
+          rv4u pk;
+          __builtin_memcpy(&pk, &av[u][0], 16);
+          __builtin_nontemporal_store(pk, reinterpret_cast<rv4u *>(col + rr));
+        } else {
+#pragma unroll
+          for (int e = 0; e < EV; ++e)
+            if (rr + e >= l && rr + e < h) col[rr + e] = av[u][e];
+        }
       }
     }
   }
