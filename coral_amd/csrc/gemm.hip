@@ -151,7 +151,8 @@ struct RealJob {
   int64_t sr, sk, R, K;
   int64_t KT;              // K tiles of the packed operand
   int64_t Rp, Kp;          // padded extents: cdiv(R, br) * br, KT * bk
-  uint32_t *mx;            // split form: max|.| bits per row (absmax writes, pack reads); may be null
+  uint32_t *mx;            // split form: max|.| bits per row (absmax or a scanning pack writes, pack reads);
+                           // may be null
   int br;                  // tile rows
   int rows_contig;         // sr == 1
   int vec;                 // base and stride allow 16-byte loads
@@ -194,10 +195,16 @@ __device__ __forceinline__ void load_run_full(const RealJob<T> &j, int64_t r, in
   __builtin_memcpy(v, &x, 16);
 }
 
-template <class TR, int BK>
+// SCAN (split forms whose 16-bit type has FP32's exponent range): the operand is
+// split unscaled and the kernel atomicMax-es each row's max |x| into mx, from the
+// block it holds in LDS.  gate (may be null): the workgroup packs only if *gate is
+// non-zero, so a re-pack can be queued behind a device-side decision.
+template <class TR, int BK, bool SCAN>
 __global__ __launch_bounds__(256) void pack_real_kernel(const RealJob<typename TR::T> ja,
-                                                        const RealJob<typename TR::T> jb) {
+                                                        const RealJob<typename TR::T> jb, const uint32_t *gate) {
   using T = typename TR::T;
+  static_assert(!SCAN || TR::SPLIT, "only split forms scan for row maxima");
+  if (gate && *gate == 0u) return;
   constexpr int LV = 16 / sizeof(T);            // elements per 16-byte load
   constexpr int VE = TR::VE, PL = TR::PL, KF = TR::KF;
   constexpr int SV = TR::SPLIT ? VE : LV;       // source elements per fragment vector
@@ -255,6 +262,23 @@ __global__ __launch_bounds__(256) void pack_real_kernel(const RealJob<typename T
   }
   __syncthreads();
 
+  if constexpr (SCAN) {  // row maxima of the block (zeros pad it beyond R and K)
+    if (j.rows_contig) {   // s[k][r]: thread t scans row t
+      float m = 0.f;
+#pragma unroll 8
+      for (int kl = 0; kl < BK; ++kl) m = fmaxf(m, fabsf((float)s[kl * ld + t]));
+      if (r0 + t < j.R) atomicMax(&j.mx[r0 + t], __float_as_uint(m));
+    } else {               // s[r][k]: four threads per row, k interleaved, then combined
+      const int rl = t >> 2, kq = t & 3;
+      float m = 0.f;
+#pragma unroll 8
+      for (int i = 0; i < KW / 4; ++i) m = fmaxf(m, fabsf((float)s[rl * ld + kq + 4 * i]));
+      m = fmaxf(m, __shfl_xor(m, 1, 64));
+      m = fmaxf(m, __shfl_xor(m, 2, 64));
+      if (kq == 0 && r0 + rl < j.R) atomicMax(&j.mx[r0 + rl], __float_as_uint(m));
+    }
+  }
+
   const int RG = j.br / 16;
   const int units = (rspan / 16) * (kspan / KF) * 64;
   for (int u = t; u < units; u += 256) {
@@ -281,7 +305,7 @@ __global__ __launch_bounds__(256) void pack_real_kernel(const RealJob<typename T
     if constexpr (!TR::SPLIT) {
       out[0] = __builtin_bit_cast(v4u, v);
     } else {
-      const int sc = (gr < j.R && j.mx) ? TR::SCALE_EXP - exp_of_max(j.mx[gr]) : 0;
+      const int sc = (!SCAN && gr < j.R && j.mx) ? TR::SCALE_EXP - exp_of_max(j.mx[gr]) : 0;
       uint32_t hi[VE], lo[VE];
 #pragma unroll
       for (int e = 0; e < VE; ++e) TR::split(ldexpf((float)v[e], sc), hi[e], lo[e]);
@@ -357,6 +381,25 @@ __global__ __launch_bounds__(256) void absmax_real_kernel(const RealJob<float> j
   }
 }
 
+// After a scanning pack: *gate = 1 if some row maximum in mxa or mxb lies
+// outside [2^-40, 2^40] and is not zero (Inf included), else 0.  Inside that range
+// the unscaled split is the scaled one times a power of two: hi, lo, every product
+// (>= 2^-80 * 2^-18 against 2^-126) and every partial sum (<= 2^31 * 2^80 against
+// 2^128) stay normal and finite, so power-of-two scaling commutes with each
+// rounding and changes no result bit.  One thread per maximum; *gate must hold 0.
+__global__ __launch_bounds__(1024) void split_gate_kernel(const uint32_t *__restrict__ mxa, int64_t na,
+                                                          const uint32_t *__restrict__ mxb, int64_t nb,
+                                                          uint32_t *__restrict__ gate) {
+  constexpr uint32_t LO = 0x2b800000u /* 2^-40 */, HI = 0x53800000u /* 2^40 */;
+  const int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+  int bad = 0;
+  if (i < na + nb) {
+    const uint32_t b = i < na ? mxa[i] : mxb[i - na];
+    bad = b != 0u && (b < LO || b > HI);
+  }
+  if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(gate, 1u);
+}
+
 // One thread writes the PL vectors (16 bytes each) of one (tile, kc, rg, lane) unit.
 template <class TR, int BR, int BK, class F>
 __global__ void pack_kernel(v4u *__restrict__ dst, int64_t R, int64_t K, int64_t KT, int64_t units,
@@ -419,6 +462,7 @@ struct GemmArgs {
   int beta_zero;
   const uint32_t *row_exp, *col_exp;  // split form only: max|.| bits per row of op(A) / column of op(B)
   int64_t re_zstride, ce_zstride;
+  const uint32_t *gate;  // split form: if set and *gate == 0, the operands were packed unscaled
 };
 
 // tile id -> (tm, tn): spread consecutive ids over the 8 XCDs so neighbours land
@@ -445,6 +489,7 @@ __device__ __forceinline__ void epilogue(const typename TR::acc_t (&acc)[FM][FN]
   using T = typename TR::T;
   const T ar = (T)g.ar, ai = (T)g.ai, br = (T)g.br, bi = (T)g.bi;
   T *C = reinterpret_cast<T *>(g.C);
+  const bool scaled = TR::SPLIT && (!g.gate || *g.gate != 0u);
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
 #pragma unroll
@@ -475,10 +520,12 @@ __device__ __forceinline__ void epilogue(const typename TR::acc_t (&acc)[FM][FN]
         } else {
           if (row < g.M && col < g.N) {
             if constexpr (TR::SPLIT) {
-              int sh = 0;
-              if (g.row_exp) sh += exp_of_max(g.row_exp[z * g.re_zstride + row]) - TR::SCALE_EXP;
-              if (g.col_exp) sh += exp_of_max(g.col_exp[z * g.ce_zstride + col]) - TR::SCALE_EXP;
-              v = ldexpf(v, sh);
+              if (scaled) {
+                int sh = 0;
+                if (g.row_exp) sh += exp_of_max(g.row_exp[z * g.re_zstride + row]) - TR::SCALE_EXP;
+                if (g.col_exp) sh += exp_of_max(g.col_exp[z * g.ce_zstride + col]) - TR::SCALE_EXP;
+                v = ldexpf(v, sh);
+              }
             }
             if constexpr (EPI == EPI_STORE_Z) {
               C[z * g.c_zstride + row + col * g.ldc] = v;
@@ -958,12 +1005,21 @@ static RealJob<T> real_job(const FetchReal<T> &f, int64_t R, int64_t K, int br, 
 }
 
 template <class TR, int BK>
-static int launch_pack_real_bk(const RealJob<typename TR::T> &ja, const RealJob<typename TR::T> &jb, hipStream_t s) {
+static int launch_pack_real_bk(const RealJob<typename TR::T> &ja, const RealJob<typename TR::T> &jb, bool scan,
+                               const uint32_t *gate, hipStream_t s) {
   if constexpr (BK % TR::KF != 0) {
     set_last_error("pack: k tile %d is not a multiple of %d", BK, TR::KF);
     return CORAL_ERR_UNSUPPORTED;
   } else {
-    hipLaunchKernelGGL((pack_real_kernel<TR, BK>), dim3((unsigned)(ja.blocks + jb.blocks)), dim3(256), 0, s, ja, jb);
+    const dim3 grid((unsigned)(ja.blocks + jb.blocks));
+    if constexpr (TR::SPLIT) {
+      if (scan) {
+        hipLaunchKernelGGL((pack_real_kernel<TR, BK, true>), grid, dim3(256), 0, s, ja, jb, gate);
+        CORAL_LAUNCH_CHECK();
+        return CORAL_OK;
+      }
+    }
+    hipLaunchKernelGGL((pack_real_kernel<TR, BK, false>), grid, dim3(256), 0, s, ja, jb, gate);
     CORAL_LAUNCH_CHECK();
     return CORAL_OK;
   }
@@ -973,12 +1029,12 @@ static int launch_pack_real_bk(const RealJob<typename TR::T> &ja, const RealJob<
 template <class TR>
 static int launch_pack_real(const TileShape &ts, v4u *pa, v4u *pb, int64_t M, int64_t N, int64_t K,
                             const FetchReal<typename TR::T> &fa, const FetchReal<typename TR::T> &fb,
-                            uint32_t *rexp, uint32_t *cexp, hipStream_t s) {
+                            uint32_t *rexp, uint32_t *cexp, bool scan, const uint32_t *gate, hipStream_t s) {
   const auto ja = real_job(fa, M, K, ts.bm, ts.bk, pa, rexp, false);
   const auto jb = real_job(fb, N, K, ts.bn, ts.bk, pb, cexp, false);
-  if (ts.bk == 16) return launch_pack_real_bk<TR, 16>(ja, jb, s);
-  if (ts.bk == 32) return launch_pack_real_bk<TR, 32>(ja, jb, s);
-  if (ts.bk == 64) return launch_pack_real_bk<TR, 64>(ja, jb, s);
+  if (ts.bk == 16) return launch_pack_real_bk<TR, 16>(ja, jb, scan, gate, s);
+  if (ts.bk == 32) return launch_pack_real_bk<TR, 32>(ja, jb, scan, gate, s);
+  if (ts.bk == 64) return launch_pack_real_bk<TR, 64>(ja, jb, scan, gate, s);
   set_last_error("pack: no kernel for k tile %d", ts.bk);
   return CORAL_ERR_UNSUPPORTED;
 }
@@ -1071,16 +1127,27 @@ static int default_tile(int64_t M, int64_t N, int64_t K) {
 
 // Shared driver for the real and complex-4M products.
 //   kind: 0 = real, 1 = complex 4M
+// gate (split real products only, may be null): rexp, cexp and *gate hold zeros; the
+// operands are packed unscaled while their row maxima are taken, split_gate_kernel
+// decides, and the scaled re-pack and the epilogue's rescale run only if it said so.
 template <class TR, class FA, class FB>
 static int run_packed(const TileShape &ts, int epi, int64_t M, int64_t N, int64_t K, FA fa, FB fb,
                       const uint32_t *rexp, const uint32_t *cexp, void *C, int64_t ldc, double ar, double ai,
-                      double br, double bi, int tile, hipStream_t s) {
+                      double br, double bi, int tile, hipStream_t s, uint32_t *gate = nullptr) {
   const int64_t KT = cdiv(K, ts.bk);
   void *pa = nullptr, *pb = nullptr;
   CORAL_TRY(workspace(s, WS_PACK_A, packed_bytes<TR>(M, K, ts.bm, ts.bk), &pa));
   CORAL_TRY(workspace(s, WS_PACK_B, packed_bytes<TR>(N, K, ts.bn, ts.bk), &pb));
   if constexpr (std::is_same<FA, FetchReal<typename TR::T>>::value) {
-    CORAL_TRY(launch_pack_real<TR>(ts, (v4u *)pa, (v4u *)pb, M, N, K, fa, fb, (uint32_t *)rexp, (uint32_t *)cexp, s));
+    if (gate) {
+      CORAL_TRY(launch_pack_real<TR>(ts, (v4u *)pa, (v4u *)pb, M, N, K, fa, fb, (uint32_t *)rexp, (uint32_t *)cexp,
+                                     true, nullptr, s));
+      hipLaunchKernelGGL(split_gate_kernel, dim3((unsigned)cdiv(M + N, 1024)), dim3(1024), 0, s, rexp, M, cexp, N,
+                         gate);
+      CORAL_LAUNCH_CHECK();
+    }
+    CORAL_TRY(launch_pack_real<TR>(ts, (v4u *)pa, (v4u *)pb, M, N, K, fa, fb, (uint32_t *)rexp, (uint32_t *)cexp,
+                                   false, gate, s));
   } else {
     CORAL_TRY(launch_pack<TR>(ts, true, (v4u *)pa, M, K, fa, rexp, 1, 0, 0, s));
     CORAL_TRY(launch_pack<TR>(ts, false, (v4u *)pb, N, K, fb, cexp, 1, 0, 0, s));
@@ -1099,6 +1166,7 @@ static int run_packed(const TileShape &ts, int epi, int64_t M, int64_t N, int64_
   g.beta_zero = (br == 0.0 && bi == 0.0);
   g.row_exp = rexp;
   g.col_exp = cexp;
+  g.gate = gate;
   if (epi == EPI_CPLX) return launch_tile<TR, EPI_CPLX>(tile, g, 1, s);
   return launch_tile<TR, EPI_REAL>(tile, g, 1, s);
 }
@@ -1180,11 +1248,18 @@ static int real_gemm(const char *routine, int32_t op_a, int32_t op_b, int64_t m,
   const TileShape ts = Menu<TR>::t()[tile];
   if constexpr (TR::SPLIT) {
     uint32_t *ex = nullptr;
-    CORAL_TRY(workspace(s, WS_AUX0, sizeof(uint32_t) * (size_t)(m + n), (void **)&ex));
-    CORAL_TRY(zero_u32(ex, m + n, s));
-    CORAL_TRY(launch_absmax_real(fa, fb, m, n, k, ex, s));
-    return run_packed<TR>(ts, EPI_REAL, m, n, k, fa, fb, ex, ex + m, c, ldc, (double)alpha, 0.0, (double)beta,
-                          0.0, tile, s);
+    CORAL_TRY(workspace(s, WS_AUX0, sizeof(uint32_t) * (size_t)(m + n + 1), (void **)&ex));
+    CORAL_TRY(zero_u32(ex, m + n + 1, s));  // row maxima, column maxima, gate
+    if constexpr (TR::SCALE_EXP == 0) {
+      // BF16 halves: FP32's exponent range, so the scaling pass is needed only for
+      // extreme rows; the maxima come from the packing pass itself
+      return run_packed<TR>(ts, EPI_REAL, m, n, k, fa, fb, ex, ex + m, c, ldc, (double)alpha, 0.0, (double)beta,
+                            0.0, tile, s, ex + m + n);
+    } else {
+      CORAL_TRY(launch_absmax_real(fa, fb, m, n, k, ex, s));
+      return run_packed<TR>(ts, EPI_REAL, m, n, k, fa, fb, ex, ex + m, c, ldc, (double)alpha, 0.0, (double)beta,
+                            0.0, tile, s);
+    }
   } else {
     return run_packed<TR>(ts, EPI_REAL, m, n, k, fa, fb, nullptr, nullptr, c, ldc, (double)alpha, 0.0,
                           (double)beta, 0.0, tile, s);

@@ -159,6 +159,52 @@ def test_split_precision_wide_range_rows(gpu_lib, which, bound):
     assert rel.max() < bound, rel.max()
 
 
+@pytest.mark.parametrize("ops", [(0, 0), (1, 1)])
+def test_bf16x3_scaled_and_unscaled_packing_agree_bitwise(gpu_lib, ops):
+    # BF16 halves are packed unscaled unless some row of op(A) or column of op(B) has
+    # its maximum outside [2^-40, 2^40] (or non-finite); then every row and column is
+    # re-packed scaled by a power of two and the epilogue scales back.  Power-of-two
+    # scaling commutes with every rounding while values stay normal, so scaling an
+    # operand by 2^60 or 2^-60 (which forces the scaled path) changes the result by
+    # exactly that factor.  NN packs A rows-contiguous and B k-contiguous; TT the
+    # other way round.
+    rng = np.random.default_rng(21)
+    oa, ob = ops
+    m, n, k = 333, 290, 203
+    lda, ldb = (m if oa == 0 else k), (k if ob == 0 else n)
+    ac, bc = (k if oa == 0 else m), (n if ob == 0 else k)
+    a = rng.standard_normal(lda * ac).astype(np.float32)
+    b = rng.standard_normal(ldb * bc).astype(np.float32)
+
+    def run(x, y):
+        out = np.zeros(m * n, np.float32)
+        blas.sgemm_bf16x3(oa, ob, m, n, k, 1.0, x, lda, y, ldb, 0.0, out, m)
+        return out
+
+    base = run(a, b)
+    big, tiny = np.float32(2.0 ** 60), np.float32(2.0 ** -60)
+    assert np.array_equal(run(a * big, b) * tiny, base)
+    assert np.array_equal(run(a, b * tiny) * big, base)
+    # one row of op(A) alone out of range: that row scales exactly, the rest are unchanged
+    a1 = a.copy()
+    row = 17
+    if oa == 0:
+        a1[row::lda] *= big
+    else:
+        a1[row * lda:row * lda + k] *= big
+    got = run(a1, b).reshape(n, m)
+    ref = base.reshape(n, m)
+    assert np.array_equal(got[:, row] * tiny, ref[:, row])
+    keep = np.arange(m) != row
+    assert np.array_equal(got[:, keep], ref[:, keep])
+    # and the result is the FP64 product to split accuracy
+    A = ref64.mat(a, m if oa == 0 else k, ac, lda)
+    B = ref64.mat(b, k if ob == 0 else n, bc, ldb)
+    ref64_c = (A.T if oa else A) @ (B.T if ob else B)
+    rel = np.abs(base.reshape(n, m).T - ref64_c).max() / (np.sqrt(k) * 2.0 ** -14)
+    assert rel < 1.0, rel
+
+
 def test_fp16x3_error_matches_plain_fp32(gpu_lib):
     # the FP16 split keeps 22 of FP32's 24 significand bits: its normwise error
     # against the FP64 product stays at the plain FP32-MFMA level (N(0,1) operands)
