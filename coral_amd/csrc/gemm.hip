@@ -107,31 +107,6 @@ __global__ void zero_u32_kernel(uint32_t *p, int64_t count) {
     p[i] = 0u;
 }
 
-// grid: x over rows, y over k chunks of `kchunk`, z over batch slots.
-template <class F>
-__global__ __launch_bounds__(256) void row_absmax_kernel(int64_t R, int64_t K, F f, uint32_t *mx, int64_t zstride,
-                                                         int kcontig, int64_t kchunk) {
-  const int z = blockIdx.z;
-  const int64_t k0 = (int64_t)blockIdx.y * kchunk;
-  const int64_t k1 = k0 + kchunk < K ? k0 + kchunk : K;
-  float m = 0.f;
-  int64_t r;
-  if (kcontig) {  // one wavefront per row, lanes walk k (coalesced)
-    const int lane = threadIdx.x & 63;
-    r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r < R)
-      for (int64_t k = k0 + lane; k < k1; k += 64) m = fmaxf(m, fabsf((float)f(r, k, z)));
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-    if (lane != 0 || r >= R) return;
-  } else {  // one thread per row, neighbouring threads read neighbouring rows (coalesced)
-    r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= R) return;
-    for (int64_t k = k0; k < k1; ++k) m = fmaxf(m, fabsf((float)f(r, k, z)));
-  }
-  atomicMax(&mx[z * zstride + r], __float_as_uint(m));
-}
-
 // ---------------------------------------------------------------------------
 // Real operands (the hot path).  One "job" per operand: element (r, k) at
 // p[r*sr + k*sk] with sr == 1 (rows contiguous: op(A) = A, op(B) = B^T) or
@@ -316,6 +291,188 @@ __global__ __launch_bounds__(256) void pack_real_kernel(const RealJob<typename T
       vl.z = lo[4] | (lo[5] << 16); vl.w = lo[6] | (lo[7] << 16);
       out[0] = vh;
       out[64] = vl;
+    }
+  }
+}
+
+// Complex 3M operands for the split forms: plane z = 0 holds Re, z = 1 Im and
+// z = 2 Re + Im of op(X) (Im negated for a conjugated operand), each packed like a
+// real operand.  A workgroup stages one block of complex elements in LDS with
+// 16-byte loads (two elements each) and writes the block's tiles of all three
+// planes, so the operand is read once: rows contiguous, 128 rows x BK k; k
+// contiguous, 64 rows x 64 k.  SCAN and gate as in pack_real_kernel, with the
+// three planes' row maxima at mx[z * mx_zstride + r]; !PACK: the maxima only.
+struct Cplx3MJob {
+  v4u *dst;
+  int64_t dst_zstride;     // v4u per plane
+  const float *p;          // interleaved (re, im)
+  int64_t sr, sk, R, K;    // element (r, k) at p[2 * (r*sr + k*sk)]
+  int64_t KT, Rp, Kp;
+  uint32_t *mx;
+  int64_t mx_zstride;
+  int br, rows_contig, vec, conj;
+  int64_t rblocks, kblocks, blocks;
+};
+
+template <class TR, int BK, bool SCAN, bool PACK>
+__global__ __launch_bounds__(256) void pack_cplx3m_kernel(const Cplx3MJob ja, const Cplx3MJob jb,
+                                                          const uint32_t *gate) {
+  static_assert(TR::SPLIT, "3M planes are packed for the split forms");
+  static_assert(SCAN || PACK, "nothing to do");
+  constexpr int VE = TR::VE, PL = TR::PL, KF = TR::KF;
+  constexpr int RW = 128, KW = 64, PAD = 1;
+  static_assert(KW % BK == 0, "k-contiguous block must hold whole K tiles");
+  constexpr int ROWS_LDS = BK * (RW + PAD), KC_LDS = 64 * (KW + PAD);
+  __shared__ float2 s[ROWS_LDS > KC_LDS ? ROWS_LDS : KC_LDS];
+  if (gate && *gate == 0u) return;
+
+  const bool first = blockIdx.x < ja.blocks;
+  const Cplx3MJob &j = first ? ja : jb;
+  const int64_t blk = first ? (int64_t)blockIdx.x : (int64_t)blockIdx.x - ja.blocks;
+  const int t = threadIdx.x;
+  int64_t r0, k0;
+  int rspan, kspan, ld;
+  if (j.rows_contig) {
+    r0 = (blk % j.rblocks) * RW;
+    k0 = (blk / j.rblocks) * BK;
+    rspan = RW; kspan = BK; ld = RW + PAD;
+    const int rl = 2 * (t & 63);
+    float2 v[BK / 4][2];
+    if (j.vec && r0 + RW <= j.R && k0 + BK <= j.K) {  // whole block inside: unguarded loads
+#pragma unroll
+      for (int c = 0; c < BK / 4; ++c) {
+        const v4u x = __builtin_nontemporal_load(
+            reinterpret_cast<const v4u *>(j.p + 2 * (r0 + rl + (k0 + (t >> 6) + 4 * c) * j.sk)));
+        __builtin_memcpy(&v[c][0], &x, 16);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < BK / 4; ++c) {
+        const int64_t k = k0 + (t >> 6) + 4 * c;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int64_t r = r0 + rl + e;
+          v[c][e] = (r < j.R && k < j.K) ? make_float2(j.p[2 * (r + k * j.sk)], j.p[2 * (r + k * j.sk) + 1])
+                                         : make_float2(0.f, 0.f);
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < BK / 4; ++c) {
+      const int kl = (t >> 6) + 4 * c;
+      s[kl * ld + rl] = v[c][0];
+      s[kl * ld + rl + 1] = v[c][1];
+    }
+  } else {
+    k0 = (blk % j.kblocks) * KW;
+    r0 = (blk / j.kblocks) * 64;
+    rspan = 64; kspan = KW; ld = KW + PAD;
+    const int kl = 2 * (t & 31);
+    float2 v[8][2];
+    if (j.vec && r0 + 64 <= j.R && k0 + KW <= j.K) {  // whole block inside: unguarded loads
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const v4u x = __builtin_nontemporal_load(
+            reinterpret_cast<const v4u *>(j.p + 2 * ((r0 + (t >> 5) + 8 * c) * j.sr + k0 + kl)));
+        __builtin_memcpy(&v[c][0], &x, 16);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int64_t r = r0 + (t >> 5) + 8 * c;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int64_t k = k0 + kl + e;
+          v[c][e] = (r < j.R && k < j.K) ? make_float2(j.p[2 * (r * j.sr + k)], j.p[2 * (r * j.sr + k) + 1])
+                                         : make_float2(0.f, 0.f);
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int rl = (t >> 5) + 8 * c;
+      s[rl * ld + kl] = v[c][0];
+      s[rl * ld + kl + 1] = v[c][1];
+    }
+  }
+  __syncthreads();
+
+  const float isg = j.conj ? -1.f : 1.f;
+  if constexpr (SCAN) {  // the three planes' row maxima (zeros pad the block)
+    float m0 = 0.f, m1 = 0.f, m2 = 0.f;
+    int64_t r = -1;
+    if (j.rows_contig) {
+      if (t < RW) {
+        for (int kl = 0; kl < BK; ++kl) {
+          const float2 x = s[kl * ld + t];
+          const float re = x.x, im = isg * x.y;
+          m0 = fmaxf(m0, fabsf(re));
+          m1 = fmaxf(m1, fabsf(im));
+          m2 = fmaxf(m2, fabsf(re + im));
+        }
+        r = r0 + t;
+      }
+    } else {
+      const int rl = t >> 2, kq = t & 3;
+#pragma unroll 4
+      for (int i = 0; i < KW / 4; ++i) {
+        const float2 x = s[rl * ld + kq + 4 * i];
+        const float re = x.x, im = isg * x.y;
+        m0 = fmaxf(m0, fabsf(re));
+        m1 = fmaxf(m1, fabsf(im));
+        m2 = fmaxf(m2, fabsf(re + im));
+      }
+#pragma unroll
+      for (int o = 1; o <= 2; o <<= 1) {
+        m0 = fmaxf(m0, __shfl_xor(m0, o, 64));
+        m1 = fmaxf(m1, __shfl_xor(m1, o, 64));
+        m2 = fmaxf(m2, __shfl_xor(m2, o, 64));
+      }
+      if (kq == 0) r = r0 + rl;
+    }
+    if (r >= 0 && r < j.R) {
+      atomicMax(&j.mx[r], __float_as_uint(m0));
+      atomicMax(&j.mx[j.mx_zstride + r], __float_as_uint(m1));
+      atomicMax(&j.mx[2 * j.mx_zstride + r], __float_as_uint(m2));
+    }
+  }
+  if constexpr (!PACK) return;
+
+  const int RG = j.br / 16;
+  const int units = (rspan / 16) * (kspan / KF) * 64;
+  for (int u = t; u < units; u += 256) {
+    const int lane = u & 63, rest = u >> 6;
+    const int rgl = rest % (rspan / 16), kcl = rest / (rspan / 16);
+    const int rl = rgl * 16 + (lane & 15), kl = kcl * KF + (lane >> 4) * VE;
+    const int64_t gr = r0 + rl, gk = k0 + kl;
+    if (gr >= j.Rp || gk >= j.Kp) continue;
+    float re[VE], im[VE];
+#pragma unroll
+    for (int e = 0; e < VE; ++e) {
+      const float2 x = j.rows_contig ? s[(kl + e) * ld + rl] : s[rl * ld + kl + e];
+      re[e] = x.x;
+      im[e] = isg * x.y;
+    }
+    const int64_t tile = (gr / j.br) * j.KT + gk / BK;
+    const int rg = (int)(gr % j.br) / 16, kc = (int)(gk % BK) / KF;
+    v4u *out = j.dst + tile * ((int64_t)(BK / KF) * RG * 64 * PL) + ((int64_t)(kc * RG + rg) * PL) * 64 +
+               ((gr & 15) + 16 * ((gk % KF) / VE));
+#pragma unroll
+    for (int z = 0; z < 3; ++z) {
+      const int sc = (!SCAN && gr < j.R && j.mx) ? TR::SCALE_EXP - exp_of_max(j.mx[z * j.mx_zstride + gr]) : 0;
+      uint32_t hi[VE], lo[VE];
+#pragma unroll
+      for (int e = 0; e < VE; ++e) {
+        const float x = z == 0 ? re[e] : (z == 1 ? im[e] : re[e] + im[e]);
+        TR::split(ldexpf(x, sc), hi[e], lo[e]);
+      }
+      v4u vh, vl;
+      vh.x = hi[0] | (hi[1] << 16); vh.y = hi[2] | (hi[3] << 16);
+      vh.z = hi[4] | (hi[5] << 16); vh.w = hi[6] | (hi[7] << 16);
+      vl.x = lo[0] | (lo[1] << 16); vl.y = lo[2] | (lo[3] << 16);
+      vl.z = lo[4] | (lo[5] << 16); vl.w = lo[6] | (lo[7] << 16);
+      out[z * j.dst_zstride] = vh;
+      out[z * j.dst_zstride + 64] = vl;
     }
   }
 }
@@ -1039,6 +1196,65 @@ static int launch_pack_real(const TileShape &ts, v4u *pa, v4u *pb, int64_t M, in
   return CORAL_ERR_UNSUPPORTED;
 }
 
+// Job for one complex 3M operand (pack_cplx3m_kernel); dst holds three planes of
+// dst_zstride vectors, mx three rows of maxima mx_zstride apart.
+static Cplx3MJob cplx3m_job(const FetchCplx3M<float> &f, int64_t R, int64_t K, int br, int bk, v4u *dst,
+                            int64_t dst_zstride, uint32_t *mx, int64_t mx_zstride) {
+  Cplx3MJob j{};
+  j.dst = dst;
+  j.dst_zstride = dst_zstride;
+  j.p = f.p;
+  j.sr = f.sr;
+  j.sk = f.sk;
+  j.R = R;
+  j.K = K;
+  j.KT = cdiv(K, bk);
+  j.Rp = cdiv(R, br) * br;
+  j.Kp = j.KT * bk;
+  j.mx = mx;
+  j.mx_zstride = mx_zstride;
+  j.br = br;
+  j.rows_contig = f.sr == 1;
+  j.conj = f.conj;
+  const int64_t stride = j.rows_contig ? f.sk : f.sr;  // in complex elements
+  j.vec = ((uintptr_t)f.p % 16 == 0) && (stride % 2 == 0);
+  j.rblocks = j.rows_contig ? cdiv(j.Rp, 128) : cdiv(j.Rp, 64);
+  j.kblocks = j.rows_contig ? j.KT : cdiv(j.Kp, 64);
+  j.blocks = j.rblocks * j.kblocks;
+  return j;
+}
+
+// mode: 0 = pack scaled by mx (if set), 1 = pack unscaled and take the maxima,
+// 2 = take the maxima only.
+template <class TR, int BK>
+static int launch_pack_3m_bk(const Cplx3MJob &ja, const Cplx3MJob &jb, int mode, const uint32_t *gate,
+                             hipStream_t s) {
+  if constexpr (BK % TR::KF != 0) {
+    set_last_error("pack: k tile %d is not a multiple of %d", BK, TR::KF);
+    return CORAL_ERR_UNSUPPORTED;
+  } else {
+    const dim3 grid((unsigned)(ja.blocks + jb.blocks));
+    if (mode == 1)
+      hipLaunchKernelGGL((pack_cplx3m_kernel<TR, BK, true, true>), grid, dim3(256), 0, s, ja, jb, gate);
+    else if (mode == 2)
+      hipLaunchKernelGGL((pack_cplx3m_kernel<TR, BK, true, false>), grid, dim3(256), 0, s, ja, jb, gate);
+    else
+      hipLaunchKernelGGL((pack_cplx3m_kernel<TR, BK, false, true>), grid, dim3(256), 0, s, ja, jb, gate);
+    CORAL_LAUNCH_CHECK();
+    return CORAL_OK;
+  }
+}
+
+// Packs the three planes of op(A) and op(B) in one launch (or takes their maxima).
+template <class TR>
+static int launch_pack_3m(const TileShape &ts, const Cplx3MJob &ja, const Cplx3MJob &jb, int mode,
+                          const uint32_t *gate, hipStream_t s) {
+  if (ts.bk == 32) return launch_pack_3m_bk<TR, 32>(ja, jb, mode, gate, s);
+  if (ts.bk == 64) return launch_pack_3m_bk<TR, 64>(ja, jb, mode, gate, s);
+  set_last_error("pack: no 3M kernel for k tile %d", ts.bk);
+  return CORAL_ERR_UNSUPPORTED;
+}
+
 // Row maxima of op(A) (m rows) and op(B) (n rows) into ex[0, m) and ex[m, m+n).
 static int launch_absmax_real(const FetchReal<float> &fa, const FetchReal<float> &fb, int64_t m, int64_t n,
                               int64_t k, uint32_t *ex, hipStream_t s) {
@@ -1066,23 +1282,6 @@ static int launch_pack(const TileShape &ts, bool is_a, v4u *dst, int64_t R, int6
     if (br == 128) return launch_pack_t<TR, 128, 64>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
     return launch_pack_t<TR, 256, 64>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
   }
-}
-
-template <class F>
-static int launch_row_max(int64_t R, int64_t K, F f, uint32_t *mx, int zdim, int64_t zstride, bool kcontig,
-                          hipStream_t s) {
-  // mx must hold zeros (see zero_u32_kernel) before the first launch
-  if (kcontig) {
-    const int64_t kchunk = 4096;
-    dim3 grid((unsigned)cdiv(R, 4), (unsigned)cdiv(K, kchunk), (unsigned)zdim);
-    hipLaunchKernelGGL(row_absmax_kernel<F>, grid, dim3(256), 0, s, R, K, f, mx, zstride, 1, kchunk);
-  } else {
-    const int64_t kchunk = 128;
-    dim3 grid((unsigned)cdiv(R, 256), (unsigned)cdiv(K, kchunk), (unsigned)zdim);
-    hipLaunchKernelGGL(row_absmax_kernel<F>, grid, dim3(256), 0, s, R, K, f, mx, zstride, 0, kchunk);
-  }
-  CORAL_LAUNCH_CHECK();
-  return CORAL_OK;
 }
 
 static int zero_u32(uint32_t *p, int64_t count, hipStream_t s) {
@@ -1309,20 +1508,31 @@ static int cgemm_3m(const char *routine, int32_t op_a, int32_t op_b, int64_t m, 
   FetchCplx3M<float> fb{b, 0, 0, op_b == CORAL_CONJUGATE_TRANSPOSE};
   strides_a<float>(op_a, lda, fa.sr, fa.sk);
   strides_b<float>(op_b, ldb, fb.sr, fb.sk);
-  // exponents: 3 x m for A, 3 x n for B; products T_z (3 x m x n fp32)
+  // row maxima: 3 x m for A, 3 x n for B, then the gate word; products T_z (3 x m x n fp32)
   uint32_t *ex = nullptr;
-  CORAL_TRY(workspace(s, WS_AUX0, sizeof(uint32_t) * 3 * (size_t)(m + n), (void **)&ex));
-  uint32_t *exa = ex, *exb = ex + 3 * m;
-  CORAL_TRY(zero_u32(ex, 3 * (m + n), s));
-  CORAL_TRY(launch_row_max(m, k, fa, exa, 3, m, fa.sk == 1, s));
-  CORAL_TRY(launch_row_max(n, k, fb, exb, 3, n, fb.sk == 1, s));
+  CORAL_TRY(workspace(s, WS_AUX0, sizeof(uint32_t) * (3 * (size_t)(m + n) + 1), (void **)&ex));
+  uint32_t *exa = ex, *exb = ex + 3 * m, *gate = nullptr;
+  CORAL_TRY(zero_u32(ex, 3 * (m + n) + 1, s));
   size_t abytes = packed_bytes<TR>(m, k, ts.bm, ts.bk), bbytes = packed_bytes<TR>(n, k, ts.bn, ts.bk);
   void *pa = nullptr, *pb = nullptr, *tz = nullptr;
   CORAL_TRY(workspace(s, WS_PACK_A, 3 * abytes, &pa));
   CORAL_TRY(workspace(s, WS_PACK_B, 3 * bbytes, &pb));
   CORAL_TRY(workspace(s, WS_AUX1, 3 * sizeof(float) * (size_t)m * (size_t)n, &tz));
-  CORAL_TRY(launch_pack<TR>(ts, true, (v4u *)pa, m, k, fa, exa, 3, (int64_t)(abytes / 16), m, s));
-  CORAL_TRY(launch_pack<TR>(ts, false, (v4u *)pb, n, k, fb, exb, 3, (int64_t)(bbytes / 16), n, s));
+  const Cplx3MJob ja = cplx3m_job(fa, m, k, ts.bm, ts.bk, (v4u *)pa, (int64_t)(abytes / 16), exa, m);
+  const Cplx3MJob jb = cplx3m_job(fb, n, k, ts.bn, ts.bk, (v4u *)pb, (int64_t)(bbytes / 16), exb, n);
+  if constexpr (TR::SCALE_EXP == 0) {
+    // BF16 halves: unscaled planes and their maxima in one pass, scaled re-pack only
+    // if the gate is set (as for the real split form)
+    gate = ex + 3 * (m + n);
+    CORAL_TRY(launch_pack_3m<TR>(ts, ja, jb, 1, nullptr, s));
+    hipLaunchKernelGGL(split_gate_kernel, dim3((unsigned)cdiv(3 * (m + n), 1024)), dim3(1024), 0, s, exa, 3 * m, exb,
+                       3 * n, gate);
+    CORAL_LAUNCH_CHECK();
+    CORAL_TRY(launch_pack_3m<TR>(ts, ja, jb, 0, gate, s));
+  } else {  // FP16 halves need the scaling: maxima first, then the scaled planes
+    CORAL_TRY(launch_pack_3m<TR>(ts, ja, jb, 2, nullptr, s));
+    CORAL_TRY(launch_pack_3m<TR>(ts, ja, jb, 0, nullptr, s));
+  }
   GemmArgs g{};
   g.Ap = (const v4u *)pa;
   g.Bp = (const v4u *)pb;
@@ -1342,6 +1552,7 @@ static int cgemm_3m(const char *routine, int32_t op_a, int32_t op_b, int64_t m, 
   g.col_exp = exb;
   g.re_zstride = m;
   g.ce_zstride = n;
+  g.gate = gate;
   CORAL_TRY((launch_tile<TR, EPI_STORE_Z>(tile, g, 3, s)));
   int64_t blocks = std::min<int64_t>(cdiv(m * n, 256), 8192);
   hipLaunchKernelGGL(combine_3m_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (const float *)tz, m, n, c, ldc,

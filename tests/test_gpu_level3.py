@@ -205,6 +205,36 @@ def test_bf16x3_scaled_and_unscaled_packing_agree_bitwise(gpu_lib, ops):
     assert rel < 1.0, rel
 
 
+@pytest.mark.parametrize("ops", [(0, 0), (2, 2), (1, 2)])
+def test_cgemm_3m_bf16x3_scaled_and_unscaled_packing_agree_bitwise(gpu_lib, ops):
+    # the 3M form packs the planes Re, Im and Re + Im of each operand the same way:
+    # unscaled unless some plane's row maximum is out of range; scaling A or B by
+    # 2^+-60 forces the scaled path and changes the result by exactly that factor
+    rng = np.random.default_rng(22)
+    oa, ob = ops
+    m, n, k = 300, 266, 197
+    lda, ldb = (m if oa == 0 else k), (k if ob == 0 else n)
+    ac, bc = (k if oa == 0 else m), (n if ob == 0 else k)
+    a = (rng.standard_normal(lda * ac) + 1j * rng.standard_normal(lda * ac)).astype(np.complex64)
+    b = (rng.standard_normal(ldb * bc) + 1j * rng.standard_normal(ldb * bc)).astype(np.complex64)
+
+    def run(x, y):
+        out = np.zeros(m * n, np.complex64)
+        blas.cgemm_3m_bf16x3(oa, ob, m, n, k, (1.0, 0.0), x, lda, y, ldb, (0.0, 0.0), out, m)
+        return out
+
+    base = run(a, b)
+    big, tiny = np.float32(2.0 ** 60), np.float32(2.0 ** -60)
+    assert np.array_equal(run(a * big, b) * tiny, base)
+    assert np.array_equal(run(a, b * tiny) * big, base)
+    A = ref64.mat(a, m if oa == 0 else k, ac, lda)
+    B = ref64.mat(b, k if ob == 0 else n, bc, ldb)
+    opa = A if oa == 0 else (A.T if oa == 1 else A.conj().T)
+    opb = B if ob == 0 else (B.T if ob == 1 else B.conj().T)
+    rel = np.abs(base.reshape(n, m).T - opa @ opb).max() / (np.sqrt(k) * 2.0 ** -12)
+    assert rel < 1.0, rel
+
+
 def test_fp16x3_error_matches_plain_fp32(gpu_lib):
     # the FP16 split keeps 22 of FP32's 24 significand bits: its normwise error
     # against the FP64 product stays at the plain FP32-MFMA level (N(0,1) operands)
