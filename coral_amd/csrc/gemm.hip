@@ -76,17 +76,13 @@ struct FetchCplxB4M {
   }
 };
 
-// Complex 3M (Gauss) operands: z = 0 -> Re, z = 1 -> Im, z = 2 -> Re + Im.
+// Complex 3M (Gauss) operand: element (r, k) at p[2 * (r*sr + k*sk)], conjugated
+// if conj.  pack_cplx3m_kernel forms its planes Re, Im and Re + Im.
 template <typename T>
 struct FetchCplx3M {
   const T *p;
   int64_t sr, sk;
   bool conj;
-  __device__ __forceinline__ T operator()(int64_t r, int64_t k, int z) const {
-    const T *e = p + 2 * (r * sr + k * sk);
-    T re = e[0], im = conj ? -e[1] : e[1];
-    return z == 0 ? re : (z == 1 ? im : re + im);
-  }
 };
 
 // ---------------------------------------------------------------------------
@@ -557,13 +553,13 @@ __global__ __launch_bounds__(1024) void split_gate_kernel(const uint32_t *__rest
   if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(gate, 1u);
 }
 
-// One thread writes the PL vectors (16 bytes each) of one (tile, kc, rg, lane) unit.
+// One thread writes the PL vectors (16 bytes each) of one (tile, kc, rg, lane) unit
+// of a plain (unsplit) operand; used for the complex 4M expansions.
 template <class TR, int BR, int BK, class F>
-__global__ void pack_kernel(v4u *__restrict__ dst, int64_t R, int64_t K, int64_t KT, int64_t units,
-                            F f, const uint32_t *__restrict__ exps, int64_t dst_zstride, int64_t exp_zstride) {
+__global__ void pack_kernel(v4u *__restrict__ dst, int64_t R, int64_t K, int64_t KT, int64_t units, F f) {
+  static_assert(!TR::SPLIT, "split operands are packed by pack_real_kernel / pack_cplx3m_kernel");
   constexpr int VE = TR::VE, PL = TR::PL, KF = TR::KF, RG = BR / 16, KC = BK / KF;
   constexpr int UNITS_PER_TILE = KC * RG * 64;
-  const int z = blockIdx.z;
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= units) return;
   int64_t tile = u / UNITS_PER_TILE;
@@ -575,31 +571,12 @@ __global__ void pack_kernel(v4u *__restrict__ dst, int64_t R, int64_t K, int64_t
   int64_t tr = tile / KT, tk = tile % KT;
   int64_t r = tr * BR + rg * 16 + (lane & 15);
   int64_t k0 = tk * BK + kc * KF + (lane >> 4) * VE;
-  v4u *out = dst + z * dst_zstride + tile * (int64_t)(UNITS_PER_TILE * PL) +
-               ((int64_t)(kc * RG + rg) * PL) * 64 + lane;
+  v4u *out = dst + tile * (int64_t)(UNITS_PER_TILE * PL) + ((int64_t)(kc * RG + rg) * PL) * 64 + lane;
   using T = typename TR::T;
-  if constexpr (!TR::SPLIT) {
-    T v[VE];
+  T v[VE];
 #pragma unroll
-    for (int t = 0; t < VE; ++t) v[t] = (r < R && k0 + t < K) ? f(r, k0 + t, z) : T(0);
-    out[0] = __builtin_bit_cast(v4u, v);
-  } else {
-    int sc = 0;
-    if (r < R && exps) sc = TR::SCALE_EXP - exp_of_max(exps[z * exp_zstride + r]);
-    uint32_t hi[VE], lo[VE];
-#pragma unroll
-    for (int t = 0; t < VE; ++t) {
-      const float x = (r < R && k0 + t < K) ? (float)f(r, k0 + t, z) : 0.f;
-      TR::split(ldexpf(x, sc), hi[t], lo[t]);  // power-of-two scaling is exact
-    }
-    v4u vh, vl;
-    vh.x = hi[0] | (hi[1] << 16); vh.y = hi[2] | (hi[3] << 16);
-    vh.z = hi[4] | (hi[5] << 16); vh.w = hi[6] | (hi[7] << 16);
-    vl.x = lo[0] | (lo[1] << 16); vl.y = lo[2] | (lo[3] << 16);
-    vl.z = lo[4] | (lo[5] << 16); vl.w = lo[6] | (lo[7] << 16);
-    out[0] = vh;
-    out[64] = vl;
-  }
+  for (int t = 0; t < VE; ++t) v[t] = (r < R && k0 + t < K) ? f(r, k0 + t, 0) : T(0);
+  out[0] = __builtin_bit_cast(v4u, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -1113,8 +1090,7 @@ static int launch_tile(int tile, const GemmArgs &g, int zdim, hipStream_t s) {
 }
 
 template <class TR, int BR, int BK, class F>
-static int launch_pack_t(v4u *dst, int64_t R, int64_t K, F f, const uint32_t *exps, int zdim,
-                         int64_t dst_zstride, int64_t exp_zstride, hipStream_t s) {
+static int launch_pack_t(v4u *dst, int64_t R, int64_t K, F f, hipStream_t s) {
   if constexpr (BK % TR::KF != 0) {
     set_last_error("pack: k tile %d is not a multiple of %d", BK, TR::KF);
     return CORAL_ERR_UNSUPPORTED;
@@ -1122,9 +1098,8 @@ static int launch_pack_t(v4u *dst, int64_t R, int64_t K, F f, const uint32_t *ex
     int64_t RT = cdiv(R, BR), KT = cdiv(K, BK);
     constexpr int UPT = (BK / TR::KF) * (BR / 16) * 64;
     int64_t units = RT * KT * UPT;
-    dim3 grid((unsigned)cdiv(units, 256), 1, (unsigned)zdim);
-    hipLaunchKernelGGL((pack_kernel<TR, BR, BK, F>), grid, dim3(256), 0, s, dst, R, K, KT, units, f, exps,
-                       dst_zstride, exp_zstride);
+    hipLaunchKernelGGL((pack_kernel<TR, BR, BK, F>), dim3((unsigned)cdiv(units, 256)), dim3(256), 0, s, dst, R, K,
+                       KT, units, f);
     CORAL_LAUNCH_CHECK();
     return CORAL_OK;
   }
@@ -1266,21 +1241,20 @@ static int launch_absmax_real(const FetchReal<float> &fa, const FetchReal<float>
 }
 
 template <class TR, class F>
-static int launch_pack(const TileShape &ts, bool is_a, v4u *dst, int64_t R, int64_t K, F f,
-                       const uint32_t *exps, int zdim, int64_t dst_zstride, int64_t exp_zstride, hipStream_t s) {
+static int launch_pack(const TileShape &ts, bool is_a, v4u *dst, int64_t R, int64_t K, F f, hipStream_t s) {
   int br = is_a ? ts.bm : ts.bn;
   if (ts.bk == 16) {
-    if (br == 64) return launch_pack_t<TR, 64, 16>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
-    if (br == 128) return launch_pack_t<TR, 128, 16>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
-    return launch_pack_t<TR, 256, 16>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
+    if (br == 64) return launch_pack_t<TR, 64, 16>(dst, R, K, f, s);
+    if (br == 128) return launch_pack_t<TR, 128, 16>(dst, R, K, f, s);
+    return launch_pack_t<TR, 256, 16>(dst, R, K, f, s);
   } else if (ts.bk == 32) {
-    if (br == 64) return launch_pack_t<TR, 64, 32>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
-    if (br == 128) return launch_pack_t<TR, 128, 32>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
-    return launch_pack_t<TR, 256, 32>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
+    if (br == 64) return launch_pack_t<TR, 64, 32>(dst, R, K, f, s);
+    if (br == 128) return launch_pack_t<TR, 128, 32>(dst, R, K, f, s);
+    return launch_pack_t<TR, 256, 32>(dst, R, K, f, s);
   } else {
-    if (br == 64) return launch_pack_t<TR, 64, 64>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
-    if (br == 128) return launch_pack_t<TR, 128, 64>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
-    return launch_pack_t<TR, 256, 64>(dst, R, K, f, exps, zdim, dst_zstride, exp_zstride, s);
+    if (br == 64) return launch_pack_t<TR, 64, 64>(dst, R, K, f, s);
+    if (br == 128) return launch_pack_t<TR, 128, 64>(dst, R, K, f, s);
+    return launch_pack_t<TR, 256, 64>(dst, R, K, f, s);
   }
 }
 
@@ -1348,8 +1322,8 @@ static int run_packed(const TileShape &ts, int epi, int64_t M, int64_t N, int64_
     CORAL_TRY(launch_pack_real<TR>(ts, (v4u *)pa, (v4u *)pb, M, N, K, fa, fb, (uint32_t *)rexp, (uint32_t *)cexp,
                                    false, gate, s));
   } else {
-    CORAL_TRY(launch_pack<TR>(ts, true, (v4u *)pa, M, K, fa, rexp, 1, 0, 0, s));
-    CORAL_TRY(launch_pack<TR>(ts, false, (v4u *)pb, N, K, fb, cexp, 1, 0, 0, s));
+    CORAL_TRY(launch_pack<TR>(ts, true, (v4u *)pa, M, K, fa, s));
+    CORAL_TRY(launch_pack<TR>(ts, false, (v4u *)pb, N, K, fb, s));
   }
   GemmArgs g{};
   g.Ap = (const v4u *)pa;
