@@ -110,7 +110,8 @@ __global__ __launch_bounds__(256) void gemv_n_kernel(int64_t m, int64_t n, const
 // flight per lane; measured at 16384^2 / 8192^2, double-precision elements stream
 // faster with one vector of eight columns (DGEMV T 6.17 -> 6.38 TB/s, ZGEMV C
 // 5.84 -> 6.32) and single-precision ones with two vectors of four columns
-// (CGEMV C 5.91 vs 5.72).
+// (CGEMV C 5.94-5.96 TB/s; 5.72 with one vector of eight columns, 5.68-5.73 with
+// two of eight, 5.88-5.90 with four of four, 5.57-5.62 with four of two).
 template <typename E>
 struct GemvTShape {
   static constexpr bool kDouble = std::is_same<typename elem_traits<E>::real, double>::value;
