@@ -162,6 +162,95 @@ __global__ __launch_bounds__(256) void symv_block_kernel(int64_t n, int upper, c
   }
 }
 
+// 16-byte elements (complex double): the same block product with the block staged
+// in LDS one half (32 columns) at a time.  Every thread's 16 loads are issued into
+// registers first, as in stream_block; only the LDS copy is split, so a workgroup
+// needs 47 KB of LDS instead of 76 KB and three fit on a CU instead of two.  Lane l
+// of wave w holds row l of columns w + 4*it (it < 16); half h holds it in
+// [8h, 8h + 8).  u (rows) accumulates over both halves in four column segments of 8
+// per half; v (columns) takes each half's 32 columns in eight row segments of 8.
+template <typename E, bool HERM>
+__global__ __launch_bounds__(256) void symv_block_halves_kernel(int64_t n, int upper, const E *__restrict__ A,
+                                                                int64_t lda, const E *__restrict__ x, int64_t incx,
+                                                                E *__restrict__ Q, int64_t nb, int vec_ok) {
+  static_assert(sizeof(E) == 16, "one element per 16-byte vector");
+  constexpr int HC = NB / 2, LDP = NB + 1, IT = NB * NB / 256;
+  __shared__ E s[HC * LDP];  // s[(c - h*HC) * LDP + r]
+  __shared__ E xr_s[NB], xc_s[NB];
+  __shared__ E up[4][NB], vp[8][NB];
+  int64_t lo, hi;
+  block_of(blockIdx.x, lo, hi);
+  const int64_t br = upper ? lo : nb - 1 - lo, bc = upper ? hi : nb - 1 - hi;
+  const int64_t r0 = br * NB, c0 = bc * NB;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const bool diag = br == bc;
+  E blk[IT];
+  if (vec_ok && c0 + NB <= n && r0 + NB <= n) {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const sv4u pk = __builtin_nontemporal_load(reinterpret_cast<const sv4u *>(A + (c0 + w + 4 * it) * lda + r0 + lane));
+      __builtin_memcpy(&blk[it], &pk, 16);
+    }
+  } else {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int64_t gr = r0 + lane, gc = c0 + w + 4 * it;
+      blk[it] = (gc < n && gr < n) ? A[gc * lda + gr] : ezero<E>();
+    }
+  }
+  if (t < NB) {
+    const int64_t i = r0 + t;
+    xr_s[t] = i < n ? x[i * incx] : ezero<E>();
+  } else if (t < 2 * NB) {
+    const int64_t j = c0 + t - NB;
+    xc_s[t - NB] = j < n ? x[j * incx] : ezero<E>();
+  }
+  const int idx = t % NB, seg = t / NB;  // u: row idx, columns h*HC + seg*8 + kk
+  const int cl = t % HC, rs = t / HC;    // v: column h*HC + cl, rows rs*8 + kk
+  E u = ezero<E>();
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (h) __syncthreads();  // every read of the first half is done
+#pragma unroll
+    for (int it = 8 * h; it < 8 * h + 8; ++it) s[(w + 4 * it - h * HC) * LDP + lane] = blk[it];
+    __syncthreads();
+    E v = ezero<E>();
+    const int c = h * HC + cl;
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      const int kc = seg * 8 + kk, k = h * HC + kc;  // column of u
+      const int r = rs * 8 + kk;                     // row of v
+      if (!diag) {
+        u = emad(u, s[kc * LDP + idx], xc_s[k]);
+        v = emad<HERM>(v, s[cl * LDP + r], xr_s[r]);
+      } else {
+        // stored half only (upper: row <= column); the diagonal counts once, in u,
+        // with its imaginary part dropped for Hermitian matrices
+        if (upper ? idx <= k : idx >= k) {
+          E a = s[kc * LDP + idx];
+          if (HERM && idx == k) a = real_part(a);
+          u = emad(u, a, xc_s[k]);
+        }
+        if (upper ? r < c : r > c) v = emad<HERM>(v, s[cl * LDP + r], xr_s[r]);
+      }
+    }
+    vp[rs][c] = v;
+  }
+  up[seg][idx] = u;
+  __syncthreads();
+  if (t < NB) {
+    const E uu = eadd(eadd(up[0][t], up[1][t]), eadd(up[2][t], up[3][t]));
+    const E vv = eadd(eadd(eadd(vp[0][t], vp[1][t]), eadd(vp[2][t], vp[3][t])),
+                      eadd(eadd(vp[4][t], vp[5][t]), eadd(vp[6][t], vp[7][t])));
+    if (diag) {
+      Q[(br * nb + bc) * NB + t] = eadd(uu, vv);
+    } else {
+      Q[(br * nb + bc) * NB + t] = uu;
+      Q[(bc * nb + br) * NB + t] = vv;
+    }
+  }
+}
+
 // y[i] := alpha * sum_c Q[r][c][w] + beta * y[i] with i = r*NB + w.  A workgroup
 // takes 64 rows; its four wavefronts sum partner ranges [g*q, (g+1)*q) in order
 // (eight loads in flight) and the group sums are added in group order, so the
@@ -234,8 +323,12 @@ static int symv_impl(int32_t uplo, int64_t n, E alpha, const E *A, int64_t lda, 
   CORAL_TRY(workspace(s, WS_AUX2, sizeof(E) * (size_t)(nb * nb * NB), &Q));
   const int64_t nblocks = nb * (nb + 1) / 2;
   const int vec_ok = ((uintptr_t)A % 16 == 0) && ((lda * (int64_t)sizeof(E)) % 16 == 0);
-  hipLaunchKernelGGL((symv_block_kernel<E, HERM>), dim3((unsigned)nblocks), dim3(256), 0, s, n,
-                     (int)(uplo == CORAL_UPPER_TRIANGULAR), A, lda, x, incx, (E *)Q, nb, vec_ok);
+  if constexpr (sizeof(E) == 16)
+    hipLaunchKernelGGL((symv_block_halves_kernel<E, HERM>), dim3((unsigned)nblocks), dim3(256), 0, s, n,
+                       (int)(uplo == CORAL_UPPER_TRIANGULAR), A, lda, x, incx, (E *)Q, nb, vec_ok);
+  else
+    hipLaunchKernelGGL((symv_block_kernel<E, HERM>), dim3((unsigned)nblocks), dim3(256), 0, s, n,
+                       (int)(uplo == CORAL_UPPER_TRIANGULAR), A, lda, x, incx, (E *)Q, nb, vec_ok);
   CORAL_LAUNCH_CHECK();
   hipLaunchKernelGGL(symv_reduce_kernel<E>, dim3((unsigned)cdiv(n, 64)), dim3(256), 0, s, n, nb, (const E *)Q, alpha,
                      beta, (int)eiszero(beta), y, incy);
