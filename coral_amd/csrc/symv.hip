@@ -162,19 +162,21 @@ __global__ __launch_bounds__(256) void symv_block_kernel(int64_t n, int upper, c
   }
 }
 
-// 16-byte elements (complex double): the same block product with the block staged
-// in LDS one half (32 columns) at a time.  Every thread's 16 loads are issued into
-// registers first, as in stream_block; only the LDS copy is split, so a workgroup
-// needs 47 KB of LDS instead of 76 KB and three fit on a CU instead of two.  Lane l
-// of wave w holds row l of columns w + 4*it (it < 16); half h holds it in
-// [8h, 8h + 8).  u (rows) accumulates over both halves in four column segments of 8
-// per half; v (columns) takes each half's 32 columns in eight row segments of 8.
+// The same block product with the block staged in LDS one half (32 columns) at a
+// time.  Every thread's loads are issued into registers first, as in stream_block;
+// only the LDS copy is split, which halves the block's LDS (complex double: 47 KB
+// per workgroup instead of 76 KB, three workgroups per CU instead of two).  Thread t
+// holds rows (t % VPC)*EV + e of column t / VPC + (256 / VPC) * it (it < IT), so
+// half h holds it in [h*IT/2, (h+1)*IT/2).  u (rows) accumulates over both halves in
+// four column segments of 8 per half; v (columns) takes each half's 32 columns in
+// eight row segments of 8.
 template <typename E, bool HERM>
 __global__ __launch_bounds__(256) void symv_block_halves_kernel(int64_t n, int upper, const E *__restrict__ A,
                                                                 int64_t lda, const E *__restrict__ x, int64_t incx,
                                                                 E *__restrict__ Q, int64_t nb, int vec_ok) {
-  static_assert(sizeof(E) == 16, "one element per 16-byte vector");
-  constexpr int HC = NB / 2, LDP = NB + 1, IT = NB * NB / 256;
+  constexpr int EV = 16 / sizeof(E), VPC = NB / EV, CPI = 256 / VPC;  // columns per it step
+  constexpr int HC = NB / 2, LDP = NB + 1, IT = NB * VPC / 256;
+  static_assert(CPI * IT / 2 == HC, "each half is half of a thread's vectors");
   __shared__ E s[HC * LDP];  // s[(c - h*HC) * LDP + r]
   __shared__ E xr_s[NB], xc_s[NB];
   __shared__ E up[4][NB], vp[8][NB];
@@ -182,20 +184,25 @@ __global__ __launch_bounds__(256) void symv_block_halves_kernel(int64_t n, int u
   block_of(blockIdx.x, lo, hi);
   const int64_t br = upper ? lo : nb - 1 - lo, bc = upper ? hi : nb - 1 - hi;
   const int64_t r0 = br * NB, c0 = bc * NB;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int t = threadIdx.x, tc = t / VPC, rv = (t % VPC) * EV;
   const bool diag = br == bc;
-  E blk[IT];
+  E blk[IT][EV];
   if (vec_ok && c0 + NB <= n && r0 + NB <= n) {
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-      const sv4u pk = __builtin_nontemporal_load(reinterpret_cast<const sv4u *>(A + (c0 + w + 4 * it) * lda + r0 + lane));
-      __builtin_memcpy(&blk[it], &pk, 16);
+      const sv4u pk =
+          __builtin_nontemporal_load(reinterpret_cast<const sv4u *>(A + (c0 + tc + CPI * it) * lda + r0 + rv));
+      __builtin_memcpy(&blk[it][0], &pk, 16);
     }
   } else {
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-      const int64_t gr = r0 + lane, gc = c0 + w + 4 * it;
-      blk[it] = (gc < n && gr < n) ? A[gc * lda + gr] : ezero<E>();
+      const int64_t gc = c0 + tc + CPI * it;
+#pragma unroll
+      for (int e = 0; e < EV; ++e) {
+        const int64_t gr = r0 + rv + e;
+        blk[it][e] = (gc < n && gr < n) ? A[gc * lda + gr] : ezero<E>();
+      }
     }
   }
   if (t < NB) {
@@ -212,7 +219,9 @@ __global__ __launch_bounds__(256) void symv_block_halves_kernel(int64_t n, int u
   for (int h = 0; h < 2; ++h) {
     if (h) __syncthreads();  // every read of the first half is done
 #pragma unroll
-    for (int it = 8 * h; it < 8 * h + 8; ++it) s[(w + 4 * it - h * HC) * LDP + lane] = blk[it];
+    for (int it = h * IT / 2; it < (h + 1) * IT / 2; ++it)
+#pragma unroll
+      for (int e = 0; e < EV; ++e) s[(tc + CPI * it - h * HC) * LDP + rv + e] = blk[it][e];
     __syncthreads();
     E v = ezero<E>();
     const int c = h * HC + cl;
@@ -323,7 +332,11 @@ static int symv_impl(int32_t uplo, int64_t n, E alpha, const E *A, int64_t lda, 
   CORAL_TRY(workspace(s, WS_AUX2, sizeof(E) * (size_t)(nb * nb * NB), &Q));
   const int64_t nblocks = nb * (nb + 1) / 2;
   const int vec_ok = ((uintptr_t)A % 16 == 0) && ((lda * (int64_t)sizeof(E)) % 16 == 0);
-  if constexpr (sizeof(E) == 16)
+  // LDS per workgroup decides how many run on a CU.  Measured, alternating runs on
+  // one box: DSYMV upper 16384 5.29 TB/s whole block in LDS, 5.58 in halves; ZHEMV
+  // lower 8192 5.0 / 5.40-5.47; SSYMV upper 16384 5.73-5.75 / 5.63-5.72 (its
+  // whole-block LDS is small already).
+  if constexpr (sizeof(E) >= 8)
     hipLaunchKernelGGL((symv_block_halves_kernel<E, HERM>), dim3((unsigned)nblocks), dim3(256), 0, s, n,
                        (int)(uplo == CORAL_UPPER_TRIANGULAR), A, lda, x, incx, (E *)Q, nb, vec_ok);
   else
