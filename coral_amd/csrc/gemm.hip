@@ -49,11 +49,12 @@ struct FetchCplxA4M {
   const T *p;
   int64_t sr, sk, K;
   bool conj;
+  // one load: the element's part is chosen by its address, the sign afterwards
   __device__ __forceinline__ T operator()(int64_t r, int64_t kk, int) const {
-    bool im = kk >= K;
-    int64_t k = im ? kk - K : kk;
-    const T *e = p + 2 * (r * sr + k * sk);
-    return im ? (conj ? -e[1] : e[1]) : e[0];
+    const bool im = kk >= K;
+    const int64_t k = im ? kk - K : kk;
+    const T v = p[2 * (r * sr + k * sk) + (im ? 1 : 0)];
+    return (im && conj) ? -v : v;
   }
 };
 
@@ -64,15 +65,18 @@ struct FetchCplxB4M {
   const T *p;
   int64_t sr, sk, K;
   bool conj;
+  // top: part 0 -> Re, part 1 -> Im'; bottom: part 0 -> -Im', part 1 -> Re, with
+  // Im' = -Im for a conjugated operand.  One load: the part is (part != bottom),
+  // negated for Im' at the bottom and for -Im at the top of a conjugated operand.
   __device__ __forceinline__ T operator()(int64_t jj, int64_t kk, int) const {
-    int64_t j = jj >> 1;
-    int part = (int)(jj & 1);
-    bool bot = kk >= K;
-    int64_t k = bot ? kk - K : kk;
-    const T *e = p + 2 * (j * sr + k * sk);
-    T re = e[0], im = conj ? -e[1] : e[1];
-    if (!bot) return part ? im : re;
-    return part ? re : -im;
+    const int64_t j = jj >> 1;
+    const bool part = jj & 1;
+    const bool bot = kk >= K;
+    const int64_t k = bot ? kk - K : kk;
+    const bool use_im = part != bot;
+    const T v = p[2 * (j * sr + k * sk) + (use_im ? 1 : 0)];
+    const bool neg = use_im && (bot != conj);
+    return neg ? -v : v;
   }
 };
 
@@ -573,9 +577,16 @@ __global__ void pack_kernel(v4u *__restrict__ dst, int64_t R, int64_t K, int64_t
   int64_t k0 = tk * BK + kc * KF + (lane >> 4) * VE;
   v4u *out = dst + tile * (int64_t)(UNITS_PER_TILE * PL) + ((int64_t)(kc * RG + rg) * PL) * 64 + lane;
   using T = typename TR::T;
+  // loads from indices clamped into the operand, all issued before any is used;
+  // elements outside it are zeroed afterwards (a guarded load per element is a
+  // branch, and each branch join waits for every load issued before it)
+  const int64_t rc = r < R ? r : R - 1;
   T v[VE];
 #pragma unroll
-  for (int t = 0; t < VE; ++t) v[t] = (r < R && k0 + t < K) ? f(r, k0 + t, 0) : T(0);
+  for (int t = 0; t < VE; ++t) v[t] = f(rc, k0 + t < K ? k0 + t : K - 1, 0);
+#pragma unroll
+  for (int t = 0; t < VE; ++t)
+    if (r >= R || k0 + t >= K) v[t] = T(0);
   out[0] = __builtin_bit_cast(v4u, v);
 }
 
