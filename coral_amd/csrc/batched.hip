@@ -97,27 +97,53 @@ __global__ __launch_bounds__(256) void batched_setup_kernel(BatchArgs g) {
   }
 }
 
-// Loads one KF-deep slice of op(A) rows [r0, r0+TS) (or op(B) columns) into LDS in
-// fragment order: element (r, kk) -> ((r/16)*64 + (r%16) + 16*(kk/VE))*VE + kk%VE.
-template <class TR, int TS>
-__device__ __forceinline__ void stage_slice(typename TR::T *lds, const typename TR::T *p, int64_t ld, bool kcontig,
-                                            int64_t r0, int64_t rmax, int64_t k0, int64_t kmax) {
+// KS KF-deep slices of op(A) rows [r0, r0+TS) (or op(B) columns) in registers.
+// load() reads them from indices clamped into the operand, so every load is
+// unconditional and all are in flight at once; store() writes them to LDS in
+// fragment order, element (r, kk) of slice s at s*TS*KF +
+// ((r/16)*64 + (r%16) + 16*(kk/VE))*VE + kk%VE, with zeros outside the operand.
+template <class TR, int TS, int KS>
+struct Slices {
   using T = typename TR::T;
-  constexpr int VE = TR::VE, KF = TR::KF, COUNT = TS * KF;
-#pragma unroll
-  for (int u = 0; u < (COUNT + 255) / 256; ++u) {
+  static constexpr int VE = TR::VE, KF = TR::KF, COUNT = TS * KF, PER = COUNT / 256;
+  static_assert(COUNT % 256 == 0, "a slice splits evenly over the workgroup");
+  T v[KS][PER];
+  __device__ __forceinline__ void coords(int u, bool kcontig, int &r, int &kk) const {
+    // neighbouring threads take neighbouring addresses of the source layout
     const int e = threadIdx.x + 256 * u;
-    if (COUNT % 256 == 0 || e < COUNT) {
-      // neighbouring threads take neighbouring addresses of the source layout
-      const int r = kcontig ? e / KF : e % TS;
-      const int kk = kcontig ? e % KF : e / TS;
-      const int64_t gr = r0 + r, gk = k0 + kk;
-      T v = T(0);
-      if (gr < rmax && gk < kmax) v = kcontig ? p[gk + gr * ld] : p[gr + gk * ld];
-      lds[((r >> 4) * 64 + (r & 15) + 16 * (kk / VE)) * VE + (kk % VE)] = v;
-    }
+    r = kcontig ? e / KF : e % TS;
+    kk = kcontig ? e % KF : e / TS;
   }
-}
+  __device__ __forceinline__ void load(const T *p, int64_t ld, bool kcontig, int64_t r0, int64_t rmax, int64_t k0,
+                                       int64_t kmax) {
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        int r, kk;
+        coords(u, kcontig, r, kk);
+        const int64_t gr = r0 + r < rmax ? r0 + r : rmax - 1;
+        const int64_t gk = k0 + s * KF + kk < kmax ? k0 + s * KF + kk : kmax - 1;
+        v[s][u] = kcontig ? p[gk + gr * ld] : p[gr + gk * ld];
+      }
+  }
+  __device__ __forceinline__ void store(T *lds, bool kcontig, int64_t r0, int64_t rmax, int64_t k0,
+                                        int64_t kmax) const {
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        int r, kk;
+        coords(u, kcontig, r, kk);
+        const bool in = r0 + r < rmax && k0 + s * KF + kk < kmax;
+        lds[s * COUNT + ((r >> 4) * 64 + (r & 15) + 16 * (kk / VE)) * VE + (kk % VE)] = in ? v[s][u] : T(0);
+      }
+  }
+};
+
+// k slices per pipeline stage: the next stage's loads are in flight while this
+// stage's MFMAs run
+constexpr int kBatchKS = 2;
 
 template <class TR, int TS>
 __device__ __forceinline__ void compute_tile(const BatchArgs &g, int64_t p, int64_t t, v4u *ldsA, v4u *ldsB) {
@@ -142,19 +168,34 @@ __device__ __forceinline__ void compute_tile(const BatchArgs &g, int64_t p, int6
   const bool a_kcontig = g.op_a != CORAL_NO_TRANSPOSE;
   const bool b_kcontig = g.op_b == CORAL_NO_TRANSPOSE;
   const int64_t kend = g.alpha == 0.0 ? 0 : K;  // alpha == 0: A and B are not read
-  for (int64_t k0 = 0; k0 < kend; k0 += TR::KF) {
-    stage_slice<TR, TS>(reinterpret_cast<T *>(ldsA), A, lda, a_kcontig, r0, m, k0, K);
-    stage_slice<TR, TS>(reinterpret_cast<T *>(ldsB), B, ldb, b_kcontig, c0, n, k0, K);
+  constexpr int KS = kBatchKS;
+  constexpr int64_t KB = (int64_t)KS * TR::KF;
+  constexpr int SLICE_V = TS * TR::KF / TR::VE;  // 16-byte vectors per slice
+  Slices<TR, TS, KS> sa, sb;
+  if (kend > 0) {
+    sa.load(A, lda, a_kcontig, r0, m, 0, K);
+    sb.load(B, ldb, b_kcontig, c0, n, 0, K);
+  }
+  for (int64_t k0 = 0; k0 < kend; k0 += KB) {
+    sa.store(reinterpret_cast<T *>(ldsA), a_kcontig, r0, m, k0, K);
+    sb.store(reinterpret_cast<T *>(ldsB), b_kcontig, c0, n, k0, K);
     __syncthreads();
-    v4u fa[F], fb[F];
+    if (k0 + KB < kend) {
+      sa.load(A, lda, a_kcontig, r0, m, k0 + KB, K);
+      sb.load(B, ldb, b_kcontig, c0, n, k0 + KB, K);
+    }
 #pragma unroll
-    for (int i = 0; i < F; ++i) fa[i] = ldsA[(wm * F + i) * 64 + lane];
+    for (int sl = 0; sl < KS; ++sl) {  // slices past K hold zeros in both operands
+      v4u fa[F], fb[F];
 #pragma unroll
-    for (int j = 0; j < F; ++j) fb[j] = ldsB[(wn * F + j) * 64 + lane];
+      for (int i = 0; i < F; ++i) fa[i] = ldsA[sl * SLICE_V + (wm * F + i) * 64 + lane];
 #pragma unroll
-    for (int i = 0; i < F; ++i)
+      for (int j = 0; j < F; ++j) fb[j] = ldsB[sl * SLICE_V + (wn * F + j) * 64 + lane];
 #pragma unroll
-      for (int j = 0; j < F; ++j) TR::mma(acc[i][j], &fa[i], &fb[j]);
+      for (int i = 0; i < F; ++i)
+#pragma unroll
+        for (int j = 0; j < F; ++j) TR::mma(acc[i][j], &fa[i], &fb[j]);
+    }
     __syncthreads();
   }
   const T al = (T)g.alpha, be = (T)g.beta;
@@ -179,7 +220,7 @@ __device__ __forceinline__ void compute_tile(const BatchArgs &g, int64_t p, int6
 
 template <class TR>
 __global__ __launch_bounds__(256) void batched_gemm_kernel(BatchArgs g) {
-  __shared__ v4u ldsA[4 * 64], ldsB[4 * 64];
+  __shared__ v4u ldsA[kBatchKS * 4 * 64], ldsB[kBatchKS * 4 * 64];
   __shared__ int s_item;
   const int64_t total = g.offsets[g.batch];
   for (;;) {
