@@ -837,7 +837,9 @@ def main():
     ap.add_argument("--gemv-n", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--parts", default="split,dgemm,dgemv,batched",
-                    help="comma list of the headline line's parts (split is always run)")
+                    help="comma list of the headline line's parts: split (the headline value; without it a "
+                         "partial line is printed), dgemm, dgemv, batched (or batched-noloop: the graph replay "
+                         "without the per-problem loop)")
     ap.add_argument("--suite", default="",
                     help="comma list of c1,c2,c3,c4,c5,rows (or one part of rows: gemv,symv,tr,rank,l1): one JSON line per configuration")
     args = ap.parse_args()
@@ -859,6 +861,21 @@ def main():
     if st != 0:
         raise SystemExit(f"coral_init failed: {lib.coral_last_error().decode()}")
     log("device:", device.device_name())
+
+    if "split" not in parts:
+        # a partial line (profiling runs): the named parts only, no headline value
+        line = {"parts": sorted(parts)}
+        if "dgemm" in parts:
+            line["dgemm_nn"] = {"n": args.n, **_dgemm_nn(args.n, args.steps)}
+        if "dgemv" in parts:
+            line["dgemv"] = {"m": args.gemv_n, "n": args.gemv_n, **_dgemv_n(args.gemv_n)}
+        if "batched" in parts or "batched-noloop" in parts:
+            for dt, key in ((np.float32, "sgemm"), (np.float64, "dgemm")):
+                r = batched_case(dt, loop="batched" in parts)
+                r.pop("_dims")
+                line[f"batched_{key}"] = r
+        print(json.dumps(line), flush=True)
+        return
 
     n = args.n
     flops = 2.0 * n ** 3

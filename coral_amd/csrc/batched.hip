@@ -50,50 +50,67 @@ struct BatchArgs {
   void *const *c;
   double alpha, beta;
   int op_a, op_b;
-  int *offsets;   // [batch + 1] prefix sums of tile counts
-  int8_t *kinds;  // [batch]
-  int *head;      // queue head
-  int table_base; // offset of this dtype / op pair in g_kind_table
+  int64_t *offsets;            // [batch + 1] prefix sums of tile counts
+  int8_t *kinds;               // [batch]
+  unsigned long long *head;    // queue head
+  int *err;                    // the stream's error word
+  int table_base;              // offset of this dtype / op pair in g_kind_table
 };
 
+// A problem is skipped (no tiles, C untouched) and flagged in the error word when a
+// dimension is negative or a leading dimension is below max(1, rows), the checks
+// the single-problem routine makes (gemm.hip check_gemm).
+__device__ __forceinline__ bool problem_valid(const BatchArgs &g, int64_t p) {
+  const int64_t m = g.m[p], n = g.n[p], k = g.k[p];
+  if (m < 0 || n < 0 || k < 0) return false;
+  const int64_t arows = g.op_a == CORAL_NO_TRANSPOSE ? m : k;
+  const int64_t brows = g.op_b == CORAL_NO_TRANSPOSE ? k : n;
+  return g.lda[p] >= imax64(1, arows) && g.ldb[p] >= imax64(1, brows) && g.ldc[p] >= imax64(1, m);
+}
+
 __global__ __launch_bounds__(256) void batched_setup_kernel(BatchArgs g) {
-  __shared__ int sums[256];
+  __shared__ int64_t sums[256];
   const int t = threadIdx.x;
   const int64_t per = (g.batch + 255) / 256;
   const int64_t p0 = t * per, p1 = p0 + per < g.batch ? p0 + per : g.batch;
-  int local = 0;
+  int64_t local = 0;
+  bool bad = false;
   for (int64_t p = p0; p < p1; ++p) {
     const int m = g.m[p], n = g.n[p], k = g.k[p];
-    int kind = 1, tiles = 0;
-    if (m > 0 && n > 0) {
+    int kind = 1;
+    int64_t tiles = 0;
+    if (!problem_valid(g, p)) {
+      bad = true;
+    } else if (m > 0 && n > 0) {
       const int idx = g.table_base + (bucket_of(m) * kNB + bucket_of(n)) * kNB + bucket_of(k > 0 ? k : 1);
       kind = g_kind_table[idx];
       if (kind < 0 || kind > 1) kind = default_kind(m, n, k);
-      const int ts = kind == 0 ? 32 : 64;
+      const int64_t ts = kind == 0 ? 32 : 64;
       tiles = ((m + ts - 1) / ts) * ((n + ts - 1) / ts);
     }
     g.kinds[p] = (int8_t)kind;
     g.offsets[p] = tiles;  // count for now
     local += tiles;
   }
+  if (bad) atomicOr(g.err, (int)ERRF_PROBLEM_ARG);
   sums[t] = local;
   __syncthreads();
   // inclusive scan of the 256 per-thread totals (Hillis-Steele)
   for (int off = 1; off < 256; off <<= 1) {
-    int v = t >= off ? sums[t - off] : 0;
+    int64_t v = t >= off ? sums[t - off] : 0;
     __syncthreads();
     sums[t] += v;
     __syncthreads();
   }
-  int run = t > 0 ? sums[t - 1] : 0;
+  int64_t run = t > 0 ? sums[t - 1] : 0;
   for (int64_t p = p0; p < p1; ++p) {
-    const int c = g.offsets[p];
+    const int64_t c = g.offsets[p];
     g.offsets[p] = run;
     run += c;
   }
   if (t == 255) {
     g.offsets[g.batch] = sums[255];
-    *g.head = 0;
+    *g.head = 0ull;
   }
 }
 
@@ -221,10 +238,10 @@ __device__ __forceinline__ void compute_tile(const BatchArgs &g, int64_t p, int6
 template <class TR>
 __global__ __launch_bounds__(256) void batched_gemm_kernel(BatchArgs g) {
   __shared__ v4u ldsA[kBatchKS * 4 * 64], ldsB[kBatchKS * 4 * 64];
-  __shared__ int s_item;
+  __shared__ int64_t s_item;
   const int64_t total = g.offsets[g.batch];
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(g.head, 1);
+    if (threadIdx.x == 0) s_item = (int64_t)atomicAdd(g.head, 1ull);
     __syncthreads();
     const int64_t item = s_item;
     __syncthreads();  // s_item is rewritten at the top of the next iteration
@@ -294,9 +311,11 @@ static int batched_impl(int dt, int32_t op_a, int32_t op_b, int64_t batch, const
   CORAL_HIP(hipStreamIsCapturing(s, &cap));
   if (cap == hipStreamCaptureStatusNone) CORAL_TRY(upload_table(s));
   void *ws = nullptr;
-  const size_t off_bytes = sizeof(int) * (size_t)(batch + 1);
+  int *err = nullptr;
+  const size_t off_bytes = sizeof(int64_t) * (size_t)(batch + 1);
   const size_t bytes = off_bytes + (size_t)batch + 64;
-  CORAL_TRY(workspace(s, WS_AUX0, bytes + 256, &ws));
+  CORAL_TRY(workspace(s, WS_BATCH, bytes + 256, &ws));
+  CORAL_TRY(error_word(s, &err));
   BatchArgs g;
   g.batch = batch;
   g.m = m; g.n = n; g.k = k; g.lda = lda; g.ldb = ldb; g.ldc = ldc;
@@ -305,9 +324,10 @@ static int batched_impl(int dt, int32_t op_a, int32_t op_b, int64_t batch, const
   g.beta = beta;
   g.op_a = op_a == CORAL_CONJUGATE_TRANSPOSE ? CORAL_TRANSPOSE : op_a;  // real: C == T
   g.op_b = op_b == CORAL_CONJUGATE_TRANSPOSE ? CORAL_TRANSPOSE : op_b;
-  g.offsets = (int *)ws;
+  g.offsets = (int64_t *)ws;
   g.kinds = (int8_t *)ws + off_bytes;
-  g.head = (int *)((char *)ws + ((off_bytes + batch + 63) & ~size_t(63)));
+  g.head = (unsigned long long *)((char *)ws + ((off_bytes + batch + 63) & ~size_t(63)));
+  g.err = err;
   g.table_base = ((dt * 4) + (g.op_a != 0) * 2 + (g.op_b != 0)) * kNB * kNB * kNB;
   hipLaunchKernelGGL(batched_setup_kernel, dim3(1), dim3(256), 0, s, g);
   CORAL_LAUNCH_CHECK();

@@ -40,9 +40,19 @@ hipStream_t resolve(coral_stream_t s);
 
 // Per-stream scratch buffers that grow on demand.  Slot ids separate the
 // buffers one routine needs at the same time.  A workspace belongs to one
-// stream; calls on one stream are ordered, so reuse across calls is safe.
-enum WsSlot { WS_PACK_A = 0, WS_PACK_B = 1, WS_AUX0 = 2, WS_AUX1 = 3, WS_AUX2 = 4, WS_SMALL = 5, WS_NSLOTS = 6 };
+// stream; calls on one stream are ordered, so reuse across calls is safe.  A slot
+// cannot grow while its stream is being captured (the call fails with
+// CORAL_ERR_UNSUPPORTED); a graph keeps every buffer its captured work uses.
+// WS_ERROR is the stream's error word (zeroed when allocated; kernels set
+// ERRF_* bits; coral_stream_check reads and clears it).
+enum WsSlot { WS_PACK_A = 0, WS_PACK_B = 1, WS_AUX0 = 2, WS_AUX1 = 3, WS_AUX2 = 4, WS_SMALL = 5, WS_ERROR = 6,
+              WS_BATCH = 7, WS_NSLOTS = 8 };
 int workspace(hipStream_t stream, int slot, size_t bytes, void **ptr);
+enum ErrFlag { ERRF_WAIT_BOUND = 1, ERRF_PROBLEM_ARG = 2 };
+int error_word(hipStream_t stream, int **ptr);
+// Spin bound of the device-side waits that have one (TRSV's staged handoff);
+// coral_testing_set_wait_limit lowers it so tests can make a wait give up.
+uint32_t wait_limit();
 
 // Optional event bracketing of dominant kernels (coral_timing_enable).
 // timing_begin returns a slot (or -1 when timing is off); timing_end closes it.

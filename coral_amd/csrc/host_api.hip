@@ -116,7 +116,9 @@ static int tr_host(size_t es, int64_t n, int64_t lda, const void *a, void *x, in
   int s = call(da ? da : a, dx ? dx : x, st);
   if (s != CORAL_OK) { finish(); return s; }
   CORAL_TRY(get(x, dx, xb));
-  return finish();
+  CORAL_TRY(finish());
+  // a TRSV whose handoff wait gave up has written an invalid x: report it
+  return coral_stream_check(st);
 }
 
 // level-1 with two vectors; y is copied back when `y_out`

@@ -58,31 +58,85 @@ int num_cus() { return g_num_cus > 0 ? g_num_cus : 256; }
 
 hipStream_t resolve(coral_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
-struct WsEntry { void *ptr[WS_NSLOTS] = {}; size_t size[WS_NSLOTS] = {}; };
+// A workspace buffer is reference counted: its stream slot holds one reference and
+// every instantiated graph whose captured work uses it holds another, so a buffer a
+// graph still reads is never freed when an eager call grows the slot or the stream
+// is destroyed; the last reference frees it.
+struct WsBuf { void *ptr = nullptr; size_t size = 0; int refs = 0; };
+struct WsEntry { WsBuf *buf[WS_NSLOTS] = {}; };
 static std::map<hipStream_t, WsEntry> g_ws;
+static std::map<hipStream_t, std::vector<WsBuf *>> g_capture_refs;  // buffers used by work being captured
+static std::map<hipGraphExec_t, std::vector<WsBuf *>> g_graph_refs;  // buffers an instantiated graph reads
+
+static void ws_release(WsBuf *b) {  // caller holds g_mu
+  if (--b->refs == 0) {
+    (void)hipFree(b->ptr);
+    delete b;
+  }
+}
+
+static bool capturing(hipStream_t stream) {
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(stream, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone;
+}
 
 int workspace(hipStream_t stream, int slot, size_t bytes, void **ptr) {
+  const bool cap = capturing(stream);
   std::lock_guard<std::mutex> lk(g_mu);
   WsEntry &w = g_ws[stream];
   if (bytes == 0) bytes = 256;
-  if (w.size[slot] < bytes) {
-    if (w.ptr[slot]) {
-      // the old buffer may still be read by queued work on this stream
+  WsBuf *b = w.buf[slot];
+  if (!b || b->size < bytes) {
+    if (cap) {
+      set_last_error("a call under stream capture needs a larger workspace than the stream holds: make the "
+                     "same call once eagerly on this stream before capturing it");
+      return CORAL_ERR_UNSUPPORTED;
+    }
+    if (b) {
+      // queued work on this stream may still read the old buffer
       CORAL_HIP(hipStreamSynchronize(stream));
-      CORAL_HIP(hipFree(w.ptr[slot]));
-      w.ptr[slot] = nullptr;
-      w.size[slot] = 0;
+      w.buf[slot] = nullptr;
+      ws_release(b);
     }
     size_t grow = bytes + bytes / 8;
     grow = (grow + 4095) & ~size_t(4095);
     void *p = nullptr;
     CORAL_HIP(hipMalloc(&p, grow));
-    w.ptr[slot] = p;
-    w.size[slot] = grow;
+    if (slot == WS_ERROR) {
+      hipError_t e = hipMemset(p, 0, grow);
+      if (e != hipSuccess) {
+        (void)hipFree(p);
+        return hip_fail(e, "hipMemset", __FILE__, __LINE__);
+      }
+    }
+    b = new WsBuf;
+    b->ptr = p;
+    b->size = grow;
+    b->refs = 1;
+    w.buf[slot] = b;
   }
-  *ptr = w.ptr[slot];
+  if (cap) {
+    std::vector<WsBuf *> &refs = g_capture_refs[stream];
+    bool seen = false;
+    for (WsBuf *r : refs) seen = seen || r == b;
+    if (!seen) {
+      ++b->refs;
+      refs.push_back(b);
+    }
+  }
+  *ptr = b->ptr;
   return CORAL_OK;
 }
+
+int error_word(hipStream_t stream, int **ptr) {
+  void *p = nullptr;
+  CORAL_TRY(workspace(stream, WS_ERROR, 256, &p));
+  *ptr = (int *)p;
+  return CORAL_OK;
+}
+
+static uint32_t g_wait_limit = 1u << 22;
+uint32_t wait_limit() { return g_wait_limit; }
 
 // ---------------------------------------------------------------------------
 // kernel timing: event pairs recorded on the launch stream
@@ -186,6 +240,8 @@ const char *coral_status_string(int s) {
     case CORAL_ERR_ALLOC: return "device allocation failed";
     case CORAL_ERR_UNSUPPORTED: return "unsupported argument combination";
     case CORAL_ERR_IO: return "autotuner file i/o failed";
+    case CORAL_ERR_WAIT_BOUND: return "a device-side wait gave up; the solve's result is invalid";
+    case CORAL_ERR_PROBLEM_ARG: return "a batched call skipped problems with invalid per-problem arguments";
     default: return "unknown status";
   }
 }
@@ -250,11 +306,40 @@ int coral_stream_destroy(coral_stream_t stream) {
     auto it = g_ws.find(resolve(stream));
     if (it != g_ws.end()) {
       for (int s = 0; s < WS_NSLOTS; ++s)
-        if (it->second.ptr[s]) (void)hipFree(it->second.ptr[s]);
+        if (it->second.buf[s]) ws_release(it->second.buf[s]);  // graphs keep what they read
       g_ws.erase(it);
     }
   }
   CORAL_HIP(hipStreamDestroy(resolve(stream)));
+  return CORAL_OK;
+}
+int coral_stream_check(coral_stream_t stream) {
+  CORAL_TRY(ensure_init());
+  hipStream_t s = resolve(stream);
+  if (capturing(s)) {
+    set_last_error("coral_stream_check synchronises the stream and cannot run under capture");
+    return CORAL_ERR_UNSUPPORTED;
+  }
+  CORAL_HIP(hipStreamSynchronize(s));
+  int *err = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_ws.find(s);
+    if (it == g_ws.end() || !it->second.buf[WS_ERROR]) return CORAL_OK;  // no checked routine ran
+    err = (int *)it->second.buf[WS_ERROR]->ptr;
+  }
+  int flags = 0;
+  CORAL_HIP(hipMemcpy(&flags, err, sizeof(int), hipMemcpyDeviceToHost));
+  if (flags == 0) return CORAL_OK;
+  CORAL_HIP(hipMemset(err, 0, sizeof(int)));
+  set_last_error("%s%s", (flags & ERRF_WAIT_BOUND) ? "a triangular solve's bounded wait gave up (its x is invalid); " : "",
+                 (flags & ERRF_PROBLEM_ARG) ? "a batched call skipped problems with invalid per-problem "
+                                              "dimensions or leading dimensions" : "");
+  return (flags & ERRF_WAIT_BOUND) ? CORAL_ERR_WAIT_BOUND : CORAL_ERR_PROBLEM_ARG;
+}
+int coral_testing_set_wait_limit(int64_t spins) {
+  if (spins < 0 || spins > (int64_t)UINT32_MAX) return -1;
+  g_wait_limit = spins == 0 ? (1u << 22) : (uint32_t)spins;
   return CORAL_OK;
 }
 int coral_stream_synchronize(coral_stream_t stream) {
@@ -297,12 +382,37 @@ int coral_graph_begin_capture(coral_stream_t stream) {
 }
 int coral_graph_end_capture(coral_stream_t stream, void **graph_exec) {
   if (!graph_exec) return -2;
+  hipStream_t s = resolve(stream);
+  std::vector<WsBuf *> refs;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_capture_refs.find(s);
+    if (it != g_capture_refs.end()) {
+      refs.swap(it->second);
+      g_capture_refs.erase(it);
+    }
+  }
+  auto drop = [&]() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (WsBuf *b : refs) ws_release(b);
+  };
   hipGraph_t graph = nullptr;
-  CORAL_HIP(hipStreamEndCapture(resolve(stream), &graph));
+  hipError_t e = hipStreamEndCapture(s, &graph);
+  if (e != hipSuccess) {
+    drop();
+    return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+  }
   hipGraphExec_t exec = nullptr;
-  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
-  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+  if (e != hipSuccess) {
+    drop();
+    return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_graph_refs[exec] = std::move(refs);
+  }
   *graph_exec = (void *)exec;
   return CORAL_OK;
 }
@@ -312,7 +422,17 @@ int coral_graph_launch(void *graph_exec, coral_stream_t stream) {
   return CORAL_OK;
 }
 int coral_graph_destroy(void *graph_exec) {
-  if (graph_exec) CORAL_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+  if (!graph_exec) return CORAL_OK;
+  hipGraphExec_t exec = (hipGraphExec_t)graph_exec;
+  CORAL_HIP(hipGraphExecDestroy(exec));
+  // the buffers are freed only after replays already queued have finished
+  CORAL_HIP(hipDeviceSynchronize());
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_graph_refs.find(exec);
+  if (it != g_graph_refs.end()) {
+    for (WsBuf *b : it->second) ws_release(b);
+    g_graph_refs.erase(it);
+  }
   return CORAL_OK;
 }
 

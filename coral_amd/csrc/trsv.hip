@@ -224,7 +224,7 @@ __device__ __forceinline__ void stage_store(typename Stage<E>::W *st, int64_t i,
 template <typename W>
 __global__ void stage_init_kernel(W *st, int64_t nwords, W sentinel, int *ctrl) {
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i0 < 2) ctrl[i0] = 0;  // [0] ticket, [1] wait-bound error flag
+  if (i0 == 0) ctrl[0] = 0;  // the ticket counter
   for (int64_t i = i0; i < nwords; i += (int64_t)gridDim.x * blockDim.x) st[i] = sentinel;
 }
 
@@ -232,7 +232,8 @@ __global__ void stage_init_kernel(W *st, int64_t nwords, W sentinel, int *ctrl) 
 // RC: the layout of M (1: trans == 0, lanes walk rows; 0: lanes walk columns).
 template <typename E, int RC>
 __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t incx, int unit,
-                                                   typename Stage<E>::W *stage, int *ctrl, int nb) {
+                                                   typename Stage<E>::W *stage, int *ctrl, int nb, int *err,
+                                                   uint32_t spin_limit) {
   __shared__ E T[NB][NB + 1];
   __shared__ E Td[NB][NB + 1];
   __shared__ E xs[NB];
@@ -302,14 +303,17 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
       if (xnext_j != j) {
         // Tickets make this wait finite (block j belongs to a workgroup that started
         // earlier and waits only on smaller tickets); the spin bound is a backstop
-        // that records an error instead of hanging if that invariant were broken.
+        // that records an error in the stream's error word (reported by
+        // coral_stream_check and by the host-pointer form) instead of hanging if that
+        // invariant were broken.  The default bound is seconds; a legitimate wait is
+        // microseconds.
         uint32_t spins = 0;
         while (!load_block(j, val)) {
-          __builtin_amdgcn_s_sleep(1);
-          if (++spins > (1u << 22)) {  // seconds; a legitimate wait is microseconds
-            if (lane == 0) atomicExch(&ctrl[1], 1);
+          if (++spins > spin_limit) {
+            if (lane == 0) atomicOr(err, (int)ERRF_WAIT_BOUND);
             break;
           }
+          __builtin_amdgcn_s_sleep(1);
         }
       }
       xs[lane] = val;
@@ -399,8 +403,10 @@ static int trsv_impl(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const
   const int nb = (int)cdiv(n, NB);
   using W = typename Stage<E>::W;
   void *ctrl = nullptr, *stage = nullptr;
-  // ctrl: [0] ticket, [1] wait-bound error flag; stage: published x, sentinel-filled
+  int *err = nullptr;
+  // ctrl: [0] ticket; stage: published x, sentinel-filled; err: the stream's error word
   CORAL_TRY(workspace(s, WS_SMALL, 2 * sizeof(int), &ctrl));
+  CORAL_TRY(error_word(s, &err));
   const int64_t nwords = n * Stage<E>::NW;
   CORAL_TRY(workspace(s, WS_AUX2, sizeof(W) * (size_t)nwords, &stage));
   hipLaunchKernelGGL(stage_init_kernel<W>, dim3((unsigned)std::min<int64_t>(cdiv(nwords, 256), 1024)), dim3(256), 0, s,
@@ -409,20 +415,25 @@ static int trsv_impl(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const
   TriView<E> v = make_view(uplo, trans, a, lda, n);
   if (v.trans == 0)
     hipLaunchKernelGGL((trsv_kernel<E, 1>), dim3((unsigned)nb), dim3(256), 0, s, v, x, incx,
-                       (int)(diag == CORAL_UNIT_DIAGONAL), (W *)stage, (int *)ctrl, nb);
+                       (int)(diag == CORAL_UNIT_DIAGONAL), (W *)stage, (int *)ctrl, nb, err, wait_limit());
   else
     hipLaunchKernelGGL((trsv_kernel<E, 0>), dim3((unsigned)nb), dim3(256), 0, s, v, x, incx,
-                       (int)(diag == CORAL_UNIT_DIAGONAL), (W *)stage, (int *)ctrl, nb);
+                       (int)(diag == CORAL_UNIT_DIAGONAL), (W *)stage, (int *)ctrl, nb, err, wait_limit());
   CORAL_LAUNCH_CHECK();
   return CORAL_OK;
 }
 
 // ---- batched DTRSV: persistent workgroups pull whole problems from a queue ----
-constexpr int kBatchMaxN = 4096;  // solution vector kept in LDS (32 KiB of doubles)
+// The working solution of a problem lives in LDS when it fits (n <= kBatchMaxN);
+// larger problems work on x in global memory (the workgroup's own writes, made
+// visible to its other waves by the barrier after each diagonal block).  A problem
+// with n < 0 or lda < max(1, n) is skipped and flagged in the stream's error word.
+constexpr int kBatchMaxN = 4096;  // 32 KiB of doubles in LDS
 
 __global__ __launch_bounds__(256) void trsv_batched_kernel(int uplo, int trans, int unit, int64_t batch,
                                                            const int32_t *ns, const double *const *As,
-                                                           const int32_t *ldas, double *const *xs_g, int *ctrl) {
+                                                           const int32_t *ldas, double *const *xs_g, int *ctrl,
+                                                           int *err) {
   __shared__ double T[NB][NB + 1];
   __shared__ double xsol[kBatchMaxN];
   __shared__ double part[4][NB];
@@ -435,17 +446,25 @@ __global__ __launch_bounds__(256) void trsv_batched_kernel(int uplo, int trans, 
     __syncthreads();
     if (p >= batch) return;  // every wavefront of the workgroup leaves together
     const int64_t n = ns[p];
-    if (n <= 0 || n > kBatchMaxN) continue;
+    const int64_t lda = ldas[p];
+    if (n == 0) continue;
+    if (n < 0 || lda < n) {
+      if (t == 0) atomicOr(err, (int)ERRF_PROBLEM_ARG);
+      continue;
+    }
     TriView<double> v;
     v.A = As[p];
-    v.lda = ldas[p];
+    v.lda = lda;
     v.n = n;
     v.trans = trans == CORAL_CONJUGATE_TRANSPOSE ? 1 : trans;
     const bool lower = uplo == CORAL_LOWER_TRIANGULAR;
     v.fwd = (lower == (trans == CORAL_NO_TRANSPOSE)) ? 1 : 0;
     double *x = xs_g[p];
-    for (int64_t i = t; i < n; i += 256) xsol[i] = x[v.map(i)];
-    __syncthreads();
+    const bool in_lds = n <= kBatchMaxN;  // uniform over the workgroup
+    if (in_lds) {
+      for (int64_t i = t; i < n; i += 256) xsol[i] = x[v.map(i)];
+      __syncthreads();
+    }
     const int nbp = (int)cdiv(n, NB);
     double reg[16];
     for (int b = 0; b < nbp; ++b) {
@@ -458,7 +477,11 @@ __global__ __launch_bounds__(256) void trsv_batched_kernel(int uplo, int trans, 
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
           const int c = q * 16 + k;
-          acc = fma(T[c][lane], xsol[j * NB + c], acc);
+          const int64_t jc = (int64_t)j * NB + c;  // < i0 <= n: a solved entry
+          double xv;
+          if (in_lds) xv = xsol[jc];
+          else xv = x[v.map(jc)];
+          acc = fma(T[c][lane], xv, acc);
         }
         __syncthreads();
       }
@@ -471,15 +494,22 @@ __global__ __launch_bounds__(256) void trsv_batched_kernel(int uplo, int trans, 
         const int nrows = (int)std::min<int64_t>(NB, n - i0);
         double rhs = 0.0, dval = 0.0;
         if (i < n) {
-          rhs = xsol[i] - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
+          double xi;
+          if (in_lds) xi = xsol[i];
+          else xi = x[v.map(i)];
+          rhs = xi - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
           if (!unit) dval = v.get(i, i);
         }
         double xr = solve_diag_block(rhs, [&](int c) { return T[c][lane]; }, dval, unit, nrows);
-        if (i < n) xsol[i] = xr;
+        if (i < n) {
+          if (in_lds) xsol[i] = xr;
+          else x[v.map(i)] = xr;
+        }
       }
       __syncthreads();
     }
-    for (int64_t i = t; i < n; i += 256) x[v.map(i)] = xsol[i];
+    if (in_lds)
+      for (int64_t i = t; i < n; i += 256) x[v.map(i)] = xsol[i];
     __syncthreads();
   }
 }
@@ -508,7 +538,7 @@ int coral_dtrsv_batched(int32_t uplo, int32_t trans, int32_t diag, int64_t batch
   if (!valid_uplo(uplo)) return -1;
   if (!valid_op(trans)) return -2;
   if (!valid_diag(diag)) return -3;
-  if (batch < 0) return -4;
+  if (batch < 0 || batch > ((int64_t)1 << 30)) return -4;
   if (batch == 0) return CORAL_OK;
   if (!n) return -5;
   if (!a) return -6;
@@ -517,12 +547,14 @@ int coral_dtrsv_batched(int32_t uplo, int32_t trans, int32_t diag, int64_t batch
   CORAL_TRY(ensure_init());
   hipStream_t s = resolve(stream);
   void *ctrl = nullptr;
+  int *err = nullptr;
   CORAL_TRY(workspace(s, WS_SMALL, 64, &ctrl));
+  CORAL_TRY(error_word(s, &err));
   hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, s, (int *)ctrl, 16);
   CORAL_LAUNCH_CHECK();
   const int64_t grid = std::min<int64_t>(batch, 2 * (int64_t)num_cus());
   hipLaunchKernelGGL(trsv_batched_kernel, dim3((unsigned)grid), dim3(256), 0, s, uplo, trans,
-                     (int)(diag == CORAL_UNIT_DIAGONAL), batch, n, a, lda, x, (int *)ctrl);
+                     (int)(diag == CORAL_UNIT_DIAGONAL), batch, n, a, lda, x, (int *)ctrl, err);
   CORAL_LAUNCH_CHECK();
   return CORAL_OK;
 }

@@ -54,7 +54,11 @@ enum {
   CORAL_ERR_NO_DEVICE = 2,  /* no usable gfx950 device                        */
   CORAL_ERR_ALLOC = 3,      /* workspace allocation failed                    */
   CORAL_ERR_UNSUPPORTED = 4,/* argument combination not supported             */
-  CORAL_ERR_IO = 5          /* autotuner table file could not be read/written */
+  CORAL_ERR_IO = 5,         /* autotuner table file could not be read/written */
+  CORAL_ERR_WAIT_BOUND = 6, /* a device-side wait gave up (coral_stream_check): the x of the
+                               triangular solve that set it is invalid                       */
+  CORAL_ERR_PROBLEM_ARG = 7 /* a batched call skipped problems whose per-problem dimensions or
+                               leading dimensions are invalid (coral_stream_check)           */
 };
 
 typedef void *coral_stream_t; /* a hipStream_t; NULL = the library's default stream */
@@ -74,6 +78,17 @@ int coral_synchronize(void);
 int coral_stream_create(coral_stream_t *stream);
 int coral_stream_destroy(coral_stream_t stream);
 int coral_stream_synchronize(coral_stream_t stream);
+/* Synchronises the stream and reports, then clears, the errors its kernels recorded
+ * since the last check: CORAL_ERR_WAIT_BOUND (a TRSV handoff wait gave up),
+ * CORAL_ERR_PROBLEM_ARG (a batched call skipped problems with invalid per-problem
+ * arguments), else CORAL_OK.  The device-pointer routines cannot report these when
+ * they return (their kernels have not run yet); the host-pointer forms check them
+ * before returning. */
+int coral_stream_check(coral_stream_t stream);
+/* Test hook: spin bound of the TRSV handoff wait (0 restores the default, about
+ * seconds of waiting); a tiny bound makes a wait give up so the error path can be
+ * exercised while every kernel still exits normally. */
+int coral_testing_set_wait_limit(int64_t spins);
 int coral_event_create(void **event);
 int coral_event_destroy(void *event);
 int coral_event_record(void *event, coral_stream_t stream);
@@ -147,9 +162,15 @@ int coral_cgemm_3m_fp16x3(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int6
 
 /* Batched uneven GEMM (no reference counterpart; equivalent to a loop of
  * coral-aarch64/src/level3/sgemm.rs:106 / dgemm.rs:107 calls).  All arrays are
- * DEVICE arrays of length `batch`: pointer arrays a/b/c and per-problem m, n, k,
- * lda, ldb, ldc.  One op pair and one alpha/beta for the whole batch.  Runs as one
- * persistent work-queue kernel; capturable in a hipGraph. */
+ * DEVICE arrays of length `batch` (at most 2^30): pointer arrays a/b/c and
+ * per-problem m, n, k, lda, ldb, ldc.  One op pair and one alpha/beta for the whole
+ * batch.  Runs as one persistent work-queue kernel; capturable in a hipGraph after
+ * one eager call of the same batch size on the capturing stream (which sizes the
+ * stream's workspace; a graph keeps the workspace it captured alive, but graphs
+ * captured from one stream share it, so replays of two such graphs must not
+ * overlap).  A problem with m, n or k < 0 or a leading dimension below its rows
+ * (max(1, rows)) is skipped, its C left unchanged, and recorded for
+ * coral_stream_check (CORAL_ERR_PROBLEM_ARG). */
 int coral_sgemm_batched(int32_t op_a, int32_t op_b, int64_t batch,
                         const int32_t *m, const int32_t *n, const int32_t *k, float alpha,
                         const float *const *a, const int32_t *lda,
@@ -222,7 +243,10 @@ int coral_ztrmv(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const doub
                 double *x, int64_t incx, coral_stream_t stream);
 
 /* Batched DTRSV (no reference counterpart; a loop of dtrsv.rs:62 calls).  Device
- * arrays of length batch: n, lda, matrix pointers, x pointers, incx = 1. */
+ * arrays of length batch (at most 2^30): n, lda, matrix pointers, x pointers,
+ * incx = 1.  Any n is solved (n <= 4096 with the solution staged in LDS, larger n
+ * in place in x).  A problem with n < 0 or lda < n is skipped, x left unchanged,
+ * and recorded for coral_stream_check (CORAL_ERR_PROBLEM_ARG). */
 int coral_dtrsv_batched(int32_t uplo, int32_t trans, int32_t diag, int64_t batch,
                         const int32_t *n, const double *const *a, const int32_t *lda,
                         double *const *x, coral_stream_t stream);

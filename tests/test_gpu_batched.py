@@ -185,3 +185,105 @@ def test_dtrsv_batched(gpu_lib):
             ref = ref64.trsv(uplo, trans, diag, n, a, n, x, 1)
             got = res[o:o + n]
             assert np.allclose(got, ref, rtol=1e-12, atol=1e-12), (n, uplo, trans, diag)
+
+
+def test_batched_graph_keeps_its_workspace(gpu_lib):
+    # A captured graph reads the stream's batched workspace (prefix sums, kinds, queue
+    # head).  Eager calls that grow the stream's workspace afterwards, and destroying
+    # the stream, must leave the graph's copy alive: the replay on another stream
+    # still gives the eager result.
+    rng = np.random.default_rng(54)
+    small = Batch(log_uniform_dims(rng, 200, 8, 128), np.float32, 0, 0, rng)
+    big = Batch(log_uniform_dims(rng, 3000, 8, 64), np.float32, 0, 0, rng)
+    s1 = device.Stream()
+    small.run(1.0, 0.0, s1)
+    s1.synchronize()
+    eager = small.results()
+    g = ctypes.c_void_p()
+    call("coral_graph_begin_capture", s1.handle)
+    small.run(1.0, 0.0, s1)
+    call("coral_graph_end_capture", s1.handle, ctypes.byref(g))
+    big.run(1.0, 0.0, s1)  # grows the batched workspace of s1
+    n = 1536  # and a split-form GEMM grows the packing and row-maximum workspaces
+    A, B, C = (DeviceArray(n * n, np.float32) for _ in range(3))
+    device.fill_random(A, 1, "normal", s1)
+    device.fill_random(B, 2, "normal", s1)
+    call("coral_sgemm_bf16x3", 0, 0, n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.0, C.ptr, n, s1.handle)
+    s1.synchronize()
+    s1.destroy()
+    s2 = device.Stream()
+    try:
+        for replay in range(3):
+            small.C.upload(np.full(small.C.count, np.nan, np.float32))
+            call("coral_graph_launch", g.value, s2.handle)
+            s2.synchronize()
+            for e, o in zip(eager, small.results()):
+                assert np.array_equal(e, o), replay
+    finally:
+        call("coral_graph_destroy", g.value)
+        s2.destroy()
+
+
+def test_batched_invalid_problem_is_skipped_and_reported(gpu_lib):
+    rng = np.random.default_rng(55)
+    dims = log_uniform_dims(rng, 30, 4, 80)
+    batch = Batch(dims, np.float64, 0, 0, rng)
+    bad = 7
+    lds = np.array(batch.ld, np.int32)
+    lda = lds[:, 0].copy()
+    lda[bad] = dims[bad][0] - 1 if dims[bad][0] > 1 else 0  # below max(1, m)
+    batch.LDA.upload(lda)
+    s = device.Stream()
+    batch.run(1.0, 0.5, s)
+    assert load_status("coral_stream_check", s.handle) == 7  # CORAL_ERR_PROBLEM_ARG
+    assert load_status("coral_stream_check", s.handle) == 0  # the check clears it
+    out = batch.results()
+    assert np.array_equal(out[bad], batch.host_c[bad])
+    batch.LDA.upload(lds[:, 0].copy())
+    ok = [i for i in range(len(dims)) if i != bad]
+    for i in ok:
+        m, n, k = dims[i]
+        ref = ref64.gemm(0, 0, m, n, k, 1.0, batch.host_a[i], batch.ld[i][0], batch.host_b[i], batch.ld[i][1], 0.5,
+                         batch.host_c[i], batch.ld[i][2])
+        assert np.allclose(out[i], ref, rtol=1e-11, atol=1e-11), i
+    s.destroy()
+
+
+def load_status(name, *args):
+    from coral_amd._lib import status_of
+
+    return status_of(name, *args)
+
+
+def test_dtrsv_batched_large_n_and_bad_lda(gpu_lib):
+    # n above the LDS-staged size is solved in place in x; a problem with lda < n is
+    # skipped (x unchanged) and reported through coral_stream_check
+    rng = np.random.default_rng(56)
+    ns = [100, 5000, 64, 40]
+    ldas = [100, 5000, 64, 39]
+    mats, xs = [], []
+    for n, lda in zip(ns, ldas):
+        A = np.diag(rng.uniform(1, 2, n)) + np.tril(rng.uniform(-1, 1, (n, n)) / n, -1)
+        mats.append(np.asfortranarray(A).reshape(-1, order="F"))  # stored with ld = n
+        xs.append(rng.uniform(-1, 1, n))
+    offs_a = np.cumsum([0] + [m.size for m in mats])[:-1]
+    offs_x = np.cumsum([0] + [x.size for x in xs])[:-1]
+    DA = DeviceArray.from_numpy(np.concatenate(mats))
+    DX = DeviceArray.from_numpy(np.concatenate(xs))
+    PA = DeviceArray.from_numpy(np.array([DA.ptr + 8 * int(o) for o in offs_a], np.uint64))
+    PX = DeviceArray.from_numpy(np.array([DX.ptr + 8 * int(o) for o in offs_x], np.uint64))
+    Ns = DeviceArray.from_numpy(np.array(ns, np.int32))
+    LDs = DeviceArray.from_numpy(np.array(ldas, np.int32))
+    s = device.Stream()
+    call("coral_dtrsv_batched", 1, 0, 1, len(ns), Ns.ptr, PA.ptr, LDs.ptr, PX.ptr, s.handle)
+    assert load_status("coral_stream_check", s.handle) == 7
+    res = DX.to_numpy()
+    for i, (n, a, x, o) in enumerate(zip(ns, mats, xs, offs_x)):
+        got = res[o:o + n]
+        if ldas[i] < n:
+            assert np.array_equal(got, x)
+            continue
+        A = a.reshape(n, n).T
+        berr = np.linalg.norm(x - A @ got) / (np.linalg.norm(A) * np.linalg.norm(got) + np.linalg.norm(x))
+        assert berr <= 4 * n * 2.0 ** -53, (n, berr)
+    s.destroy()

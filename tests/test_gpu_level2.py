@@ -194,3 +194,39 @@ def test_dtrsv_n8192_backward_error(gpu_lib):
     blas.dtrsv(1, 0, 1, n, a, n, x, 1)
     berr = np.linalg.norm(b - A @ x) / (np.linalg.norm(A) * np.linalg.norm(x) + np.linalg.norm(b))
     assert berr <= 2.0 ** -53 * 16, berr
+
+
+def test_trsv_wait_bound_is_reported(gpu_lib):
+    # The host lowers the spin bound of TRSV's handoff wait to one retry, so waits
+    # give up (the kernels still run to the end); the host-pointer form and
+    # coral_stream_check must report CORAL_ERR_WAIT_BOUND, and with the default
+    # bound restored the same solve is correct and clean.
+    from coral_amd._lib import CoralError, call, status_of
+    from coral_amd.device import DeviceArray, Stream
+
+    rng = np.random.default_rng(24)
+    n = 16384
+    A = np.tril(rng.uniform(-1, 1, (n, n)) / n, -1) + np.diag(rng.uniform(1, 2, n))
+    a = np.asfortranarray(A).reshape(-1, order="F")
+    b = rng.uniform(-1, 1, n)
+    call("coral_testing_set_wait_limit", 1)
+    try:
+        x = b.copy()
+        with pytest.raises(CoralError) as e:
+            blas.dtrsv(1, 0, 1, n, a, n, x, 1)
+        assert e.value.status == 6  # CORAL_ERR_WAIT_BOUND
+        dA, dx, s = DeviceArray.from_numpy(a), DeviceArray.from_numpy(b), Stream()
+        call("coral_dtrsv", 1, 0, 1, n, dA.ptr, n, dx.ptr, 1, s.handle)
+        assert status_of("coral_stream_check", s.handle) == 6
+        assert status_of("coral_stream_check", s.handle) == 0
+    finally:
+        call("coral_testing_set_wait_limit", 0)
+    dx.upload(b)
+    call("coral_dtrsv", 1, 0, 1, n, dA.ptr, n, dx.ptr, 1, s.handle)
+    assert status_of("coral_stream_check", s.handle) == 0
+    x = dx.to_numpy()
+    s.destroy()
+    berr = np.linalg.norm(b - A @ x) / (np.linalg.norm(A) * np.linalg.norm(x) + np.linalg.norm(b))
+    assert berr <= 2.0 ** -53 * 16, berr
+    x = b.copy()
+    blas.dtrsv(1, 0, 1, n, a, n, x, 1)  # the host form is clean again
