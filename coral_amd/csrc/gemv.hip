@@ -18,6 +18,7 @@
 #include <type_traits>
 
 #include <algorithm>
+#include <mutex>
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -229,10 +230,18 @@ int launch_beta_only(int64_t len, E beta, E *y, int64_t incy, hipStream_t s) {
   return CORAL_OK;
 }
 
+// Partial-sum buffer of a GEMV call: the stream's workspace, or (fixed != nullptr) a
+// caller-provided buffer of cap bytes, in which case the chunk count is cut to fit
+// and nothing is allocated.
+struct GemvScratch {
+  void *fixed = nullptr;
+  size_t cap = 0;
+};
+
 // CORAL argument order: trans=1 m=2 n=3 alpha=4 a=5 lda=6 x=7 incx=8 beta=9 y=10 incy=11
 template <typename E>
 int gemv_impl(int32_t trans, int64_t m, int64_t n, E alpha, const E *A, int64_t lda, const E *x, int64_t incx,
-              E beta, E *y, int64_t incy, coral_stream_t stream) {
+              E beta, E *y, int64_t incy, coral_stream_t stream, GemvScratch fs = GemvScratch()) {
   if (!valid_op(trans)) return -1;
   if (m < 0) return -2;
   if (n < 0) return -3;
@@ -267,11 +276,12 @@ int gemv_impl(int32_t trans, int64_t m, int64_t n, E alpha, const E *A, int64_t 
     const int64_t row_blocks = cdiv(m, rows_per_wg);
     // enough column chunks for cfg_wg workgroups per CU, each chunk >= 32 columns
     int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(cdiv((int64_t)cfg_wg * cus, row_blocks), cdiv(n, 32)));
+    if (fs.fixed) chunks = std::min<int64_t>(chunks, (int64_t)(fs.cap / (sizeof(E) * (size_t)m)));
     const int64_t cpc = cdiv(n, chunks);
     chunks = cdiv(n, cpc);
-    void *P = nullptr;
-    CORAL_TRY(workspace(s, WS_AUX2, sizeof(E) * (size_t)chunks * (size_t)m, &P));
-    const int slot = timing_begin(TK_GEMV, s);
+    void *P = fs.fixed;
+    if (!P) CORAL_TRY(workspace(s, WS_AUX2, sizeof(E) * (size_t)chunks * (size_t)m, &P));
+    const int slot = fs.fixed ? -1 : timing_begin(TK_GEMV, s);
     if (cfg_nv == 4)
       hipLaunchKernelGGL((gemv_n_kernel<E, 4>), dim3((unsigned)row_blocks, (unsigned)chunks), dim3(256), 0, s, m, n,
                          A, lda, x, incx, (E *)P, cpc, (int)a_aligned);
@@ -287,12 +297,13 @@ int gemv_impl(int32_t trans, int64_t m, int64_t n, E alpha, const E *A, int64_t 
     const int64_t col_blocks = cdiv(n, 4 * GemvTShape<E>::CPW);
     const int64_t unit = 64 * GemvTShape<E>::NV * (16 / (int64_t)sizeof(E));  // rows per wave step
     int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(cdiv(8 * cus, col_blocks), cdiv(m, 4 * unit)));
+    if (fs.fixed) chunks = std::min<int64_t>(chunks, (int64_t)(fs.cap / (sizeof(E) * (size_t)n)));
     const int64_t rpc = cdiv(cdiv(m, chunks), unit) * unit;
     chunks = cdiv(m, rpc);
     const bool x_aligned = ((uintptr_t)x % 16 == 0);
-    void *P = nullptr;
-    CORAL_TRY(workspace(s, WS_AUX2, sizeof(E) * (size_t)chunks * (size_t)n, &P));
-    const int slot = timing_begin(TK_GEMV, s);
+    void *P = fs.fixed;
+    if (!P) CORAL_TRY(workspace(s, WS_AUX2, sizeof(E) * (size_t)chunks * (size_t)n, &P));
+    const int slot = fs.fixed ? -1 : timing_begin(TK_GEMV, s);
     if (trans == CORAL_CONJUGATE_TRANSPOSE && elem_traits<E>::cplx)
       hipLaunchKernelGGL((gemv_t_kernel<E, true>), dim3((unsigned)col_blocks, (unsigned)chunks), dim3(256), 0, s, m, n,
                          A, lda, x, incx, (E *)P, rpc, (int)a_aligned, (int)x_aligned);
@@ -306,6 +317,57 @@ int gemv_impl(int32_t trans, int64_t m, int64_t n, E alpha, const E *A, int64_t 
     CORAL_LAUNCH_CHECK();
   }
   return CORAL_OK;
+}
+
+// Scratch of the allocation-free DGEMV (coral_dgemv_noalloc): the partial sums of its
+// column chunks (N) or row chunks (T), at most kNoAllocScratch bytes, so y may hold
+// at most kNoAllocScratch / 8 = 2^19 elements.  One copy per GPU, held until the
+// process ends; every such call on a GPU shares it (calls that overlap on one GPU
+// race on it).  Host code uses only the device address hipGetSymbolAddress returns
+// for the current device, never the array's name.
+constexpr size_t kNoAllocScratch = size_t(4) << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratch];
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_address(void **ptr) {
+  int dev = 0;
+  CORAL_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_last_error("device %d is beyond the %d devices the allocation-free GEMV supports", dev, kMaxDevices);
+    return CORAL_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    CORAL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return CORAL_OK;
+}
+
+int dgemv_noalloc(int32_t trans, int64_t m, int64_t n, double alpha, const double *a, int64_t lda,
+                  const double *x, int64_t incx, double beta, double *y, int64_t incy, coral_stream_t stream) {
+  if (!valid_op(trans)) return -1;
+  const bool notrans = trans == CORAL_NO_TRANSPOSE;
+  // the partial sums of one chunk cover all of y
+  if (notrans && m > (int64_t)(kNoAllocScratch / sizeof(double))) {
+    set_last_error("coral_dgemv_noalloc: m = %lld exceeds the %zu rows its fixed scratch holds", (long long)m,
+                   kNoAllocScratch / sizeof(double));
+    return -2;
+  }
+  if (!notrans && n > (int64_t)(kNoAllocScratch / sizeof(double))) {
+    set_last_error("coral_dgemv_noalloc: n = %lld exceeds the %zu columns its fixed scratch holds", (long long)n,
+                   kNoAllocScratch / sizeof(double));
+    return -3;
+  }
+  CORAL_TRY(ensure_init());
+  GemvScratch fs;
+  CORAL_TRY(scratch_address(&fs.fixed));
+  fs.cap = kNoAllocScratch;
+  return gemv_impl<double>(trans, m, n, alpha, a, lda, x, incx, beta, y, incy, stream, fs);
 }
 
 template int launch_beta_only<float>(int64_t, float, float *, int64_t, hipStream_t);
@@ -328,6 +390,10 @@ int coral_sgemv(int32_t trans, int64_t m, int64_t n, float alpha, const float *a
 int coral_dgemv(int32_t trans, int64_t m, int64_t n, double alpha, const double *a, int64_t lda, const double *x,
                 int64_t incx, double beta, double *y, int64_t incy, coral_stream_t stream) {
   return gemv_impl<double>(trans, m, n, alpha, a, lda, x, incx, beta, y, incy, stream);
+}
+int coral_dgemv_noalloc(int32_t trans, int64_t m, int64_t n, double alpha, const double *a, int64_t lda,
+                        const double *x, int64_t incx, double beta, double *y, int64_t incy, coral_stream_t stream) {
+  return dgemv_noalloc(trans, m, n, alpha, a, lda, x, incx, beta, y, incy, stream);
 }
 int coral_cgemv(int32_t trans, int64_t m, int64_t n, const float alpha[2], const float *a, int64_t lda,
                 const float *x, int64_t incx, const float beta[2], float *y, int64_t incy, coral_stream_t stream) {

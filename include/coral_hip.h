@@ -211,6 +211,17 @@ int coral_cgemv(int32_t trans, int64_t m, int64_t n, const float alpha[2], const
 int coral_zgemv(int32_t trans, int64_t m, int64_t n, const double alpha[2], const double *a, int64_t lda,
                 const double *x, int64_t incx, const double beta[2], double *y, int64_t incy, coral_stream_t stream);
 
+/* Allocation-free DGEMV: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops, hosts
+ * that forbid allocation): the same arguments, kernels and results as coral_dgemv,
+ * with the partial sums in a fixed 4 MiB device array of the library (one per GPU,
+ * held until the process ends) instead of a workspace; it calls no hipMalloc.  y
+ * may hold at most 2^19 elements (m for NoTranspose, n otherwise); larger calls
+ * return -2 (m) or -3 (n).  Every call on one GPU shares the array: calls that
+ * overlap on one GPU (on different streams) race. */
+int coral_dgemv_noalloc(int32_t trans, int64_t m, int64_t n, double alpha, const double *a, int64_t lda,
+                        const double *x, int64_t incx, double beta, double *y, int64_t incy, coral_stream_t stream);
+
 /* coral-aarch64/src/level2/ssymv.rs:99 / dsymv.rs:99 */
 int coral_ssymv(int32_t uplo, int64_t n, float alpha, const float *a, int64_t lda,
                 const float *x, int64_t incx, float beta, float *y, int64_t incy, coral_stream_t stream);
