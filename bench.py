@@ -779,6 +779,16 @@ def _split_forms(n, steps, warmup):
     return res
 
 
+def _autotune_table():
+    """The tile table libcoralhip loaded at start (CORAL_AUTOTUNE_TABLE, else the one
+    shipped beside the library), or None."""
+    env = os.environ.get("CORAL_AUTOTUNE_TABLE", "")
+    if env == "none":
+        return None
+    path = env or os.path.join(ROOT, "coral_amd", "autotune_table.txt")
+    return path if os.path.exists(path) else None
+
+
 def _dgemm_nn(n, steps):
     from coral_amd import device, synthetic
     from coral_amd._lib import call
@@ -854,7 +864,7 @@ def main():
     parts = {p.strip() for p in args.parts.split(",") if p.strip()}
 
     from coral_amd import device
-    from coral_amd._lib import load
+    from coral_amd._lib import call, load
 
     lib = load()
     st = lib.coral_init()
@@ -939,15 +949,24 @@ def main():
         line["dgemv"] = {"m": args.gemv_n, "n": args.gemv_n, **_dgemv_n(args.gemv_n)}
     if "batched" in parts:
         bl = {}
+        table = _autotune_table()
         for dt, key in ((np.float32, "sgemm"), (np.float64, "dgemm")):
             r = batched_case(dt)
             r.pop("_dims")
             r["meets_target"] = bool(r["speedup"] >= 5.0)
+            if table:
+                # the same batch with the shape heuristic in place of the tuned table
+                call("coral_autotune_clear")
+                h = batched_case(dt, loop=False)
+                call("coral_autotune_load", table.encode())
+                r["heuristic_graph_ms"] = h["graph_ms"]
+                r["tuned_over_heuristic"] = round(h["graph_ms"] / r["graph_ms"], 3)
             bl[key] = r
             log(f"batched {key} 10000 problems: graph {r['graph_ms']:.3f} ms, loop {r['loop_ms']:.1f} ms, "
                 f"x{r['speedup']:.1f}")
         line["batched_vs_launch_loop"] = {"workload": "10000 NN problems, m, n, k log-uniform integers in [8,512], "
                                                       "N(0,1), one hipGraph replay vs a per-problem launch loop",
+                                          "tile_table": os.path.relpath(table, ROOT) if table else None,
                                           "target_speedup": 5.0, **bl}
     if not args.no_cpu_baseline:
         log("cpu baseline (CORAL SGEMM restatement, 1 core) ...")

@@ -7,6 +7,8 @@
 // ceil(log2 k)).  Lookups fall back to a shape heuristic when no entry exists.
 #include "coral_common.hpp"
 
+#include <dlfcn.h>
+
 #include <map>
 #include <mutex>
 #include <stdio.h>
@@ -137,6 +139,29 @@ static int tune_batched(const char *routine, int32_t op_a, int32_t op_b, int64_t
 }  // namespace coral
 
 using namespace coral;
+
+namespace coral {
+// The table the library starts with: the file CORAL_AUTOTUNE_TABLE names ("none":
+// no table), else autotune_table.txt beside libcoralhip.so when it exists.
+int autotune_load_default() {
+  const char *env = getenv("CORAL_AUTOTUNE_TABLE");
+  std::string path;
+  if (env && *env) {
+    if (!strcmp(env, "none")) return CORAL_OK;
+    path = env;
+  } else {
+    Dl_info info;
+    if (!dladdr((void *)&autotune_load_default, &info) || !info.dli_fname) return CORAL_OK;
+    path = info.dli_fname;
+    const size_t slash = path.rfind('/');
+    path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/autotune_table.txt";
+    FILE *probe = fopen(path.c_str(), "r");
+    if (!probe) return CORAL_OK;  // no table shipped: heuristics
+    fclose(probe);
+  }
+  return coral_autotune_load(path.c_str());
+}
+}  // namespace coral
 
 extern "C" {
 

@@ -75,6 +75,7 @@ inline bool valid_op(int32_t op) { return op == CORAL_NO_TRANSPOSE || op == CORA
 inline bool valid_uplo(int32_t u) { return u == CORAL_UPPER_TRIANGULAR || u == CORAL_LOWER_TRIANGULAR; }
 inline bool valid_diag(int32_t d) { return d == CORAL_UNIT_DIAGONAL || d == CORAL_NON_UNIT_DIAGONAL; }
 __host__ __device__ inline int64_t imax64(int64_t a, int64_t b) { return a > b ? a : b; }
+__host__ __device__ inline int64_t imin64(int64_t a, int64_t b) { return a < b ? a : b; }
 __host__ __device__ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // ---------------------------------------------------------------------------

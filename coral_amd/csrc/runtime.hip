@@ -12,6 +12,8 @@
 
 namespace coral {
 
+int autotune_load_default();  // autotune.hip
+
 static thread_local char g_last_error[512] = "";
 static std::mutex g_mu;
 static int g_device = -1;
@@ -50,6 +52,8 @@ int ensure_init() {
     return CORAL_ERR_NO_DEVICE;
   }
   g_num_cus = prop.multiProcessorCount;
+  // a shipped tile table that cannot be read leaves the heuristics in place
+  (void)autotune_load_default();
   g_device = dev;
   return CORAL_OK;
 }

@@ -168,9 +168,9 @@ int coral_cgemm_3m_fp16x3(int32_t op_a, int32_t op_b, int64_t m, int64_t n, int6
  * one eager call of the same batch size on the capturing stream (which sizes the
  * stream's workspace; a graph keeps the workspace it captured alive, but graphs
  * captured from one stream share it, so replays of two such graphs must not
- * overlap).  A problem with m, n or k < 0 or a leading dimension below its rows
- * (max(1, rows)) is skipped, its C left unchanged, and recorded for
- * coral_stream_check (CORAL_ERR_PROBLEM_ARG). */
+ * overlap).  A problem with m, n or k < 0, a leading dimension below its rows
+ * (max(1, rows)), or lda or ldb above 2^24 is skipped, its C left unchanged, and
+ * recorded for coral_stream_check (CORAL_ERR_PROBLEM_ARG). */
 int coral_sgemm_batched(int32_t op_a, int32_t op_b, int64_t batch,
                         const int32_t *m, const int32_t *n, const int32_t *k, float alpha,
                         const float *const *a, const int32_t *lda,

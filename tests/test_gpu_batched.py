@@ -239,6 +239,16 @@ def test_batched_invalid_problem_is_skipped_and_reported(gpu_lib):
     assert load_status("coral_stream_check", s.handle) == 0  # the check clears it
     out = batch.results()
     assert np.array_equal(out[bad], batch.host_c[bad])
+    # lda above 2^24 (the tile kernel's 32-bit offsets) is refused the same way; the
+    # matrix is never read
+    big = 11
+    lda2 = lds[:, 0].copy()
+    lda2[big] = (1 << 24) + 1
+    batch.LDA.upload(lda2)
+    batch.C.upload(np.concatenate(batch.host_c))
+    batch.run(1.0, 0.5, s)
+    assert load_status("coral_stream_check", s.handle) == 7
+    assert np.array_equal(batch.results()[big], batch.host_c[big])
     batch.LDA.upload(lds[:, 0].copy())
     ok = [i for i in range(len(dims)) if i != bad]
     for i in ok:
