@@ -230,3 +230,44 @@ def test_trsv_wait_bound_is_reported(gpu_lib):
     assert berr <= 2.0 ** -53 * 16, berr
     x = b.copy()
     blas.dtrsv(1, 0, 1, n, a, n, x, 1)  # the host form is clean again
+
+
+@pytest.mark.parametrize("p,uplo,trans,n,scale", [
+    ("d", 1, 0, 8192, "4/sqrt(n)"), ("d", 0, 1, 8192, "4/sqrt(n)"), ("d", 1, 0, 1024, "1"), ("d", 0, 0, 1024, "1"),
+    ("z", 1, 2, 4096, "4/sqrt(n)"), ("s", 1, 0, 4096, "4/sqrt(n)"),
+])
+def test_trsv_not_diagonally_dominant(gpu_lib, p, uplo, trans, n, scale):
+    # Triangles whose off-diagonal rows outweigh the diagonal (U[-1,1] * 4/sqrt(n):
+    # |row sum| ~ 2 sqrt(n) against a diagonal in [1, 2]), and, for n = 1024, plain
+    # U[-1,1] off the diagonal: condition about 1e18 with |x| near 1e28.  The GPU
+    # solves each 64-row diagonal block with its explicitly inverted block; its
+    # backward error |b - op(A) x| / (|A| |x| + |b|) is held to that of CORAL's NB = 8
+    # substitution (the oracle) on the same system.
+    rng = np.random.default_rng(25 + n + uplo + 3 * trans)
+    f = {"s": np.float32, "d": np.float64, "z": np.complex128}[p]
+    eps = 2.0 ** -24 if p == "s" else 2.0 ** -53
+    s = 4.0 / np.sqrt(n) if scale != "1" else 1.0
+    off = rng.uniform(-1, 1, (n, n)) * s
+    if p == "z":
+        off = off + 1j * rng.uniform(-1, 1, (n, n)) * s
+    A = (np.tril(off, -1) if uplo == 1 else np.triu(off, 1)) + np.diag(rng.uniform(1, 2, n))
+    A = A.astype(f)
+    a = np.asfortranarray(A).reshape(-1, order="F")
+    b = rng.uniform(-1, 1, n).astype(f)
+    if p == "z":
+        b = (b + 1j * rng.uniform(-1, 1, n)).astype(f)
+    x = b.copy()
+    getattr(blas, f"{p}trsv")(uplo, trans, 1, n, a, n, x, 1)
+    xo = b.copy()
+    getattr(port, f"{p}trsv")(uplo, trans, 1, n, a, n, xo, 1)
+    wide = np.complex128 if p == "z" else np.float64
+    M = A.astype(wide)
+    M = M if trans == 0 else (M.T if trans == 1 else M.conj().T)
+
+    def berr(v):
+        v = v.astype(wide)
+        return np.linalg.norm(b.astype(wide) - M @ v) / (np.linalg.norm(M) * np.linalg.norm(v) + np.linalg.norm(b))
+
+    assert np.all(np.isfinite(x))
+    bg, bo = berr(x), berr(xo)
+    assert bg <= 16 * max(bo, eps), (bg, bo)

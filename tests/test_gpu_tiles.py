@@ -78,6 +78,10 @@ def test_every_menu_tile(gpu_lib, routine):
                     assert np.array_equal(pad, c.reshape(n, ldc)[:, m:]), (routine, t)
     finally:
         lib.coral_autotune_clear()
+        shipped = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "coral_amd",
+                               "autotune_table.txt")
+        if os.path.exists(shipped):  # the table the library loaded at start
+            assert lib.coral_autotune_load(shipped.encode()) == 0
 
 
 def test_autotune_run_save_load_roundtrip(gpu_lib, tmp_path):
@@ -107,3 +111,7 @@ def test_autotune_run_save_load_roundtrip(gpu_lib, tmp_path):
         assert np.linalg.norm(out - ref) <= (2.0 ** -16 + k * 2.0 ** -24) * np.linalg.norm(a) * np.linalg.norm(b)
     finally:
         lib.coral_autotune_clear()
+        shipped = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "coral_amd",
+                               "autotune_table.txt")
+        if os.path.exists(shipped):  # the table the library loaded at start
+            assert lib.coral_autotune_load(shipped.encode()) == 0
