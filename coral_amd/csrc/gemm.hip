@@ -1029,16 +1029,19 @@ template <int BM_, int BN_, int BK_, int WVM_ = 0, int WVN_ = 0, int SCH_ = 0> s
 };
 template <class... S> struct ShapeList {};
 
-// Tile menus per element kind; the index is the tile id the autotuner stores.
+// Tile menus per element kind; the index is the tile id the autotuner stores.  No
+// entry spills registers: a 256 x 256 tile needs eight waves (4 waves would hold
+// 64 accumulator tiles each), so the 4-wave entries are at most 128 x 256 for the
+// split forms.
 using MenuF64 = ShapeList<Shape<128, 128, 16>, Shape<64, 64, 16>, Shape<128, 128, 32>, Shape<128, 128, 16, 2, 2>,
                           Shape<128, 128, 16, 2, 4>, Shape<64, 64, 32, 2, 2>, Shape<128, 64, 32, 2, 2>,
                           Shape<256, 128, 16, 2, 4>, Shape<128, 128, 16, 2, 2, 2>, Shape<128, 128, 32, 2, 4, 2>>;
 using MenuF32 = ShapeList<Shape<128, 128, 32>, Shape<64, 64, 32>, Shape<256, 128, 32>, Shape<256, 256, 32>,
-                          Shape<128, 128, 64>, Shape<256, 256, 32, 2, 2>, Shape<256, 256, 32, 2, 4>,
+                          Shape<128, 128, 64>, Shape<128, 256, 32, 2, 4>, Shape<256, 256, 32, 2, 4>,
                           Shape<128, 128, 64, 2, 2>, Shape<256, 128, 32, 2, 2>, Shape<256, 256, 32, 2, 4, 1>,
                           Shape<256, 256, 32, 4, 2, 1>, Shape<256, 256, 32, 2, 4, 2>, Shape<256, 256, 32, 4, 2, 2>>;
-using MenuBF3 = ShapeList<Shape<128, 128, 32>, Shape<64, 64, 32>, Shape<256, 128, 32>, Shape<256, 256, 32>,
-                          Shape<128, 128, 64>, Shape<256, 256, 32, 2, 2>, Shape<256, 256, 32, 2, 4>,
+using MenuBF3 = ShapeList<Shape<128, 128, 32>, Shape<64, 64, 32>, Shape<256, 128, 32>, Shape<128, 256, 32>,
+                          Shape<128, 128, 64>, Shape<128, 256, 32, 2, 4>, Shape<256, 256, 32, 2, 4>,
                           Shape<128, 128, 64, 2, 2>, Shape<256, 128, 32, 2, 2>, Shape<256, 256, 32, 2, 4, 1>,
                           Shape<256, 256, 32, 4, 2, 1>, Shape<256, 256, 32, 2, 4, 2>, Shape<256, 256, 32, 4, 2, 2>>;
 

@@ -323,7 +323,9 @@ static int scal_impl(int64_t n, A alpha, E *x, int64_t incx, coral_stream_t stre
 
 // ---------------------------------------------------------------------------
 // nrm2.  Single precision: squares summed in double (no overflow/underflow is
-// possible for binary32 inputs), then sqrt and one rounding.  Double precision:
+// possible for binary32 inputs), then sqrt and one rounding -- the one use of FP64
+// in a single-precision routine: the kernel streams x at the HBM rate, so the FP64
+// FMAs cost no time, and it needs neither scaling nor a second accumulator.  Double precision:
 // Blue's three-accumulator scaled sum of squares (one pass, no division; the
 // thresholds are those of Anderson, "A Fresh Look at Blue's Algorithm", 2017).
 // NaN anywhere gives NaN and Inf gives Inf (IEEE), where CORAL's chunked scaling
@@ -435,32 +437,31 @@ static int nrm2_impl(int64_t n, const E *x, int64_t incx, R *result, coral_strea
 // keeps the larger value and, on equal values, the smaller index, which makes the
 // result independent of the reduction order.
 // ---------------------------------------------------------------------------
-struct ValIdx { double v; int64_t i; };
+// Magnitudes stay in the element's precision (float for s/c, double for d/z).
+template <typename R> struct ValIdx { R v; int64_t i; };
 
-__device__ __forceinline__ ValIdx vi_best(ValIdx a, ValIdx b) {
+template <typename R>
+__device__ __forceinline__ ValIdx<R> vi_best(ValIdx<R> a, ValIdx<R> b) {
   if (b.v > a.v || (b.v == a.v && b.i < a.i)) return b;
   return a;
 }
 
 template <typename E>
-__device__ __forceinline__ double mag(E v) {
-  if constexpr (elem_traits<E>::cplx) {
-    typename elem_traits<E>::real m = fabs(v.x) + fabs(v.y);  // rounded in element precision
-    return (double)m;
-  } else {
-    return (double)fabs(v);
-  }
+__device__ __forceinline__ typename elem_traits<E>::real mag(E v) {
+  if constexpr (elem_traits<E>::cplx) return fabs(v.x) + fabs(v.y);  // rounded in element precision
+  else return fabs(v);
 }
 
 template <typename E>
 __global__ __launch_bounds__(kThreads) void iamax_partial_kernel(int64_t n, const E *__restrict__ x, int64_t incx,
-                                                                ValIdx *part, int vec) {
-  __shared__ ValIdx sh[kThreads];
-  ValIdx b{-1.0, INT64_MAX};  // -1: nothing seen (magnitudes are >= 0)
+                                                                ValIdx<typename elem_traits<E>::real> *part, int vec) {
+  using R = typename elem_traits<E>::real;
+  __shared__ ValIdx<R> sh[kThreads];
+  ValIdx<R> b{R(-1), INT64_MAX};  // -1: nothing seen (magnitudes are >= 0)
   // each thread visits its indices in increasing order, so the strict '>' keeps the
   // first index of a maximum per thread
   stream_visit<E, false>(n, x, incx, x, incx, vec, [&](int64_t i, E v, E) {
-    const double m = mag(v);
+    const R m = mag(v);
     if (m == m && m > b.v) { b.v = m; b.i = i; }
   });
   sh[threadIdx.x] = b;
@@ -472,9 +473,10 @@ __global__ __launch_bounds__(kThreads) void iamax_partial_kernel(int64_t n, cons
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-__global__ __launch_bounds__(kThreads) void iamax_final_kernel(const ValIdx *part, int64_t np, int64_t *result) {
-  __shared__ ValIdx sh[kThreads];
-  ValIdx b{-1.0, INT64_MAX};
+template <typename R>
+__global__ __launch_bounds__(kThreads) void iamax_final_kernel(const ValIdx<R> *part, int64_t np, int64_t *result) {
+  __shared__ ValIdx<R> sh[kThreads];
+  ValIdx<R> b{R(-1), INT64_MAX};
   for (int64_t i = threadIdx.x; i < np; i += kThreads) b = vi_best(b, part[i]);
   sh[threadIdx.x] = b;
   __syncthreads();
@@ -482,7 +484,7 @@ __global__ __launch_bounds__(kThreads) void iamax_final_kernel(const ValIdx *par
     if (threadIdx.x < w) sh[threadIdx.x] = vi_best(sh[threadIdx.x], sh[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) *result = (sh[0].v < 0.0) ? 0 : sh[0].i;
+  if (threadIdx.x == 0) *result = (sh[0].v < R(0)) ? 0 : sh[0].i;
 }
 
 template <typename E>
@@ -498,13 +500,14 @@ static int iamax_impl(int64_t n, const E *x, int64_t incx, int64_t *result, cora
     return CORAL_OK;
   }
   if (!x) return -2;
+  using R = typename elem_traits<E>::real;
   const int64_t g = reduce_grid(n);
   void *part = nullptr;
-  CORAL_TRY(workspace(s, WS_SMALL, sizeof(ValIdx) * (size_t)g, &part));
-  hipLaunchKernelGGL(iamax_partial_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, s, n, x, incx, (ValIdx *)part,
+  CORAL_TRY(workspace(s, WS_SMALL, sizeof(ValIdx<R>) * (size_t)g, &part));
+  hipLaunchKernelGGL(iamax_partial_kernel<E>, dim3((unsigned)g), dim3(kThreads), 0, s, n, x, incx, (ValIdx<R> *)part,
                      (int)contiguous16(x, incx));
   CORAL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(iamax_final_kernel, dim3(1), dim3(kThreads), 0, s, (const ValIdx *)part, g, result);
+  hipLaunchKernelGGL(iamax_final_kernel<R>, dim3(1), dim3(kThreads), 0, s, (const ValIdx<R> *)part, g, result);
   CORAL_LAUNCH_CHECK();
   return CORAL_OK;
 }

@@ -12,6 +12,11 @@ Fixtures are data only: inputs and expected outputs.
   of sgemm_tests.rs:151-156 and op pair, with the expected C from numpy float64 /
   complex128 matmul on the rounded inputs (numpy's OpenBLAS is the same library
   family as the reference tests' CBLAS oracle, coral-aarch64/Cargo.toml:25-31).
+* ``gemm_reference_shapes.npz`` -- the larger reference shapes (128^3 and the
+  rectangular 64x192x15 / 192x64x15, sgemm_tests.rs:221-235, cgemm_tests.rs:260-274)
+  for every run of the reference's run_all_ops, as signatures: the expected C at
+  192 fixed sampled entries and its Frobenius norm (inputs come from the same
+  generators, so tests rebuild them).
 * ``iamax_cases.npz`` -- the planted-maximum vectors of
   coral-aarch64/tests/level1/iamax_tests.rs:79-140 with their expected indices.
 """
@@ -47,15 +52,31 @@ SHAPES = [  # (name, m, n, k, lda_n, lda_t, ldb_n, ldb_t, ldc)
     ("padded_lds", 17, 19, 13, 20, 15, 18, 23, 24),
 ]
 REAL_CASES = [(1.0, 0.0), (0.5, 1.0), (0.75, -0.25), (0.0, 0.7)]
-CPLX_CASES = [((1.0, 0.0), (0.0, 0.0)), ((0.5, -0.25), (1.0, 0.0)), ((0.75, 0.1), (-0.25, 0.5)),
+# (alpha, beta) lists of sgemm_tests.rs:151-156 and cgemm_tests.rs:196-201
+CPLX_CASES = [((1.0, 0.0), (0.0, 0.0)), ((0.5, 0.25), (1.0, 0.0)), ((0.75, -0.25), (-0.5, 0.3)),
               ((0.0, 0.0), (0.7, -0.2))]
+# the larger reference shapes (sgemm_tests.rs:221-235, cgemm_tests.rs:260-274), kept
+# as sampled signatures: their inputs come from the same generators
+LARGE_SHAPES = [  # (name, m, n, k, lda_n, lda_t, ldb_n, ldb_t, ldc)
+    ("medium_square", 128, 128, 128, 128, 128, 128, 128, 128),
+    ("rectangular_wide", 64, 192, 15, 64, 15, 15, 192, 64),
+    ("rectangular_tall", 192, 64, 15, 192, 15, 15, 64, 192),
+]
+SAMPLES = 192
 
 
 def colmajor(rows, cols, ld, f, dtype):
+    """Column-major buffer with element (i, j) = f(i, j), evaluated elementwise in the
+    generator's own precision (f receives integer-valued arrays of that type)."""
     a = np.zeros(ld * cols, dtype)
-    for j in range(cols):
-        for i in range(rows):
-            a[i + j * ld] = f(i, j)
+    real = np.float32 if np.dtype(dtype) in (np.float32, np.complex64) else np.float64
+    i = np.arange(rows, dtype=real)[:, None]
+    j = np.arange(cols, dtype=real)[None, :]
+    vals = f(i, j)
+    if isinstance(vals, tuple):
+        vals = vals[0] + 1j * vals[1]
+    for c in range(cols):
+        a[c * ld: c * ld + rows] = np.broadcast_to(vals, (rows, cols))[:, c]
     return a
 
 
@@ -90,12 +111,12 @@ def cplx_case(dt, m, n, k, oa, ob, lda, ldb, ldc, alpha, beta):
     cd = np.complex64 if dt == "c" else np.complex128
     ar, ac = (m, k) if oa == 0 else (k, m)
     br, bc = (k, n) if ob == 0 else (n, k)
-    a = colmajor(ar, ac, lda, lambda i, j: complex(f(0.1) + f(i) * f(0.25) + f(j) * f(0.125),
-                                                   f(-0.05) + f(i) * f(0.20) - f(j) * f(0.075)), cd)
-    b = colmajor(br, bc, ldb, lambda i, j: complex(f(-0.2) + f(i) * f(0.05) - f(j) * f(0.075),
-                                                   f(0.15) - f(i) * f(0.03) + f(j) * f(0.02)), cd)
-    c = colmajor(m, n, ldc, lambda i, j: complex(f(0.3) - f(i) * f(0.01) + f(j) * f(0.02),
-                                                 f(-0.1) + f(i) * f(0.015) - f(j) * f(0.025)), cd)
+    a = colmajor(ar, ac, lda, lambda i, j: (f(0.1) + f(i) * f(0.25) + f(j) * f(0.125),
+                                            f(-0.05) + f(i) * f(0.20) - f(j) * f(0.075)), cd)
+    b = colmajor(br, bc, ldb, lambda i, j: (f(-0.2) + f(i) * f(0.05) - f(j) * f(0.075),
+                                            f(0.15) - f(i) * f(0.03) + f(j) * f(0.02)), cd)
+    c = colmajor(m, n, ldc, lambda i, j: (f(0.3) - f(i) * f(0.01) + f(j) * f(0.02),
+                                          f(-0.1) + f(i) * f(0.015) - f(j) * f(0.025)), cd)
     A = view(a, ar, ac, lda).astype(np.complex128)
     B = view(b, br, bc, ldb).astype(np.complex128)
     C = view(c, m, n, ldc).astype(np.complex128)
@@ -136,6 +157,59 @@ def gemm_cases():
     return len(index)
 
 
+def reference_runs(dt):
+    """(case index, op_a, op_b) of one run_all_ops call: real routines run the four
+    N/T pairs per (alpha, beta) plus ConjTranspose x NoTranspose with the first case
+    and NoTranspose x ConjTranspose with the second (sgemm_tests.rs:158-185); complex
+    routines run all nine pairs per case (cgemm_tests.rs:203-224)."""
+    runs = []
+    for ci in range(4):
+        if dt in "cz":
+            runs += [(ci, oa, ob) for oa in (0, 1, 2) for ob in (0, 1, 2)]
+        else:
+            runs += [(ci, oa, ob) for oa in (0, 1) for ob in (0, 1)] + [(0, 2, 0), (1, 0, 2)]
+    return runs
+
+
+def large_case(dt, name, m, n, k, lda_n, lda_t, ldb_n, ldb_t, ldc, ci, oa, ob):
+    """Inputs of one larger reference run and its expected C (float64 / complex128)."""
+    lda = lda_n if oa == 0 else lda_t
+    ldb = ldb_n if ob == 0 else ldb_t
+    if dt in "cz":
+        alpha, beta = CPLX_CASES[ci]
+        a, b, c, out = cplx_case(dt, m, n, k, oa, ob, lda, ldb, ldc, alpha, beta)
+    else:
+        alpha, beta = REAL_CASES[ci]
+        a, b, c, out = real_case(dt, m, n, k, oa, ob, lda, ldb, ldc, alpha, beta)
+    return lda, ldb, alpha, beta, a, b, c, out
+
+
+def sample_positions(m, n, ldc):
+    """Fixed buffer offsets i + j ldc of the sampled entries of an m x n result."""
+    rng = np.random.default_rng(m * 100000 + n * 100 + ldc)
+    cells = rng.choice(m * n, min(SAMPLES, m * n), replace=False)
+    return np.sort((cells % m) + (cells // m) * ldc)
+
+
+def large_gemm_cases():
+    data, index = {}, []
+    for dt in "sdcz":
+        for (name, m, n, k, lda_n, lda_t, ldb_n, ldb_t, ldc) in LARGE_SHAPES:
+            pos = sample_positions(m, n, ldc)
+            data[f"{name}_pos"] = pos
+            for ci, oa, ob in reference_runs(dt):
+                lda, ldb, alpha, beta, a, b, c, out = large_case(dt, name, m, n, k, lda_n, lda_t, ldb_n, ldb_t,
+                                                                 ldc, ci, oa, ob)
+                key = f"{dt}_{name}_{ci}_{oa}{ob}"
+                data[key + "_sample"] = out[pos]
+                region = np.stack([out[j * ldc: j * ldc + m] for j in range(n)], axis=1)
+                data[key + "_fro"] = np.float64(np.linalg.norm(region))
+                index.append([key, dt, name, ci, oa, ob, m, n, k, lda, ldb, ldc])
+    data["index"] = np.array(json.dumps(index))
+    np.savez_compressed(os.path.join(HERE, "gemm_reference_shapes.npz"), **data)
+    return len(index)
+
+
 def iamax_cases():
     n1 = 1024
     k = np.arange(n1, dtype=np.float32)
@@ -156,5 +230,6 @@ if __name__ == "__main__":
     with open(os.path.join(HERE, "kat.json"), "w") as f:
         json.dump(KAT, f, indent=1)
     print("gemm cases:", gemm_cases())
+    print("larger reference gemm shapes:", large_gemm_cases())
     iamax_cases()
     print("wrote", sorted(p for p in os.listdir(HERE) if not p.endswith(".py")))

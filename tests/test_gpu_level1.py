@@ -114,6 +114,11 @@ def test_iamax_bit_exact(gpu_lib):
         vc = (v + 1j * v[::-1]).astype(np.complex128)
         assert blas.izamax(n, vc, 1) == port.izamax(n, vc, 1), n
         assert blas.icamax(n, vc.astype(np.complex64), 1) == port.icamax(n, vc.astype(np.complex64), 1), n
+        if n <= 100000:  # brute force (first index of the largest |x| or |re| + |im|) for every variant
+            assert blas.isamax(n, vf, 1) == ref64.iamax(vf, n, 1), n
+            assert blas.izamax(n, vc, 1) == ref64.iamax(vc, n, 1, cplx=True), n
+            assert blas.icamax(n, vc.astype(np.complex64), 1) == ref64.iamax(vc.astype(np.complex64), n, 1,
+                                                                             cplx=True), n
     v = rng.uniform(-1, 1, 3000)
     assert blas.idamax(1000, v, 3) == port.idamax(1000, v, 3)
 

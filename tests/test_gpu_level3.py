@@ -67,6 +67,27 @@ def test_golden_reference_generators(gpu_lib):
         assert allclose_ref(out, po, *GEMM_TOL[dt]), key
 
 
+def test_golden_larger_reference_shapes(gpu_lib):
+    # the reference tests' 128^3 and rectangular 64x192x15 / 192x64x15 runs, inputs
+    # from their generators, against the committed signatures and the full FP64
+    # product at the reference tolerances, and against the oracle
+    from tests.test_oracle import large_reference_cases
+
+    count = 0
+    for key, dt, (m, n, k, oa, ob, lda, ldb, ldc, alpha, beta), (a, b, c), out, pos, sample, fro in \
+            large_reference_cases():
+        got = c.copy()
+        GPU_FN[dt](oa, ob, m, n, k, alpha, a, lda, b, ldb, beta, got, ldc)
+        rtol, atol = GEMM_TOL[dt]
+        assert allclose_ref(got[pos], sample, rtol, atol), key
+        assert allclose_ref(got, out, rtol, atol), key
+        po = c.copy()
+        PORT_FN[dt](oa, ob, m, n, k, alpha, a, lda, b, ldb, beta, po, ldc)
+        assert allclose_ref(got, po, rtol, atol), key
+        count += 1
+    assert count == 360
+
+
 SHAPES = [(1, 1, 1), (5, 7, 4), (64, 64, 64), (127, 129, 65), (300, 257, 129), (17, 513, 40), (512, 16, 777)]
 
 

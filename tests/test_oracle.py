@@ -56,6 +56,41 @@ def test_gemm_golden_reference_generators():
         assert allclose_ref(c, expect, rtol, atol), key
 
 
+def large_reference_cases():
+    """Runs of the larger reference GEMM shapes (128^3, 64x192x15, 192x64x15) with
+    their inputs rebuilt from the reference generators and the committed signature
+    (expected C at fixed sampled entries, and its Frobenius norm)."""
+    from tests.golden import make_golden as mg
+
+    z = np.load(os.path.join(GOLD, "gemm_reference_shapes.npz"))
+    shapes = {s[0]: s for s in mg.LARGE_SHAPES}
+    for key, dt, name, ci, oa, ob, m, n, k, lda, ldb, ldc in json.loads(str(z["index"])):
+        _, _, _, _, lda_n, lda_t, ldb_n, ldb_t, _ = shapes[name]
+        lda_, ldb_, alpha, beta, a, b, c, out = mg.large_case(dt, name, m, n, k, lda_n, lda_t, ldb_n, ldb_t, ldc, ci,
+                                                              oa, ob)
+        assert (lda_, ldb_) == (lda, ldb)
+        yield key, dt, (m, n, k, oa, ob, lda, ldb, ldc, alpha, beta), (a, b, c), out, z[f"{name}_pos"], \
+            z[key + "_sample"], float(z[key + "_fro"])
+
+
+def test_gemm_larger_reference_shapes():
+    fns = {"s": port.sgemm, "d": port.dgemm, "c": port.cgemm, "z": port.zgemm}
+    count = 0
+    for key, dt, (m, n, k, oa, ob, lda, ldb, ldc, alpha, beta), (a, b, c), out, pos, sample, fro in \
+            large_reference_cases():
+        # the rebuilt reference result matches the committed signature
+        assert np.allclose(out[pos], sample, rtol=1e-14, atol=1e-14), key
+        region = np.stack([out[j * ldc: j * ldc + m] for j in range(n)], axis=1)
+        assert abs(np.linalg.norm(region) - fro) <= 1e-12 * fro, key
+        cc = c.copy()
+        fns[dt](oa, ob, m, n, k, alpha, a, lda, b, ldb, beta, cc, ldc)
+        rtol, atol = GEMM_TOL[dt]
+        assert allclose_ref(cc, out, rtol, atol), key
+        assert allclose_ref(cc[pos], sample, rtol, atol), key
+        count += 1
+    assert count == 360
+
+
 def test_gemm_exact_rational_tiny():
     # dyadic inputs: the port's f64 result must equal the exact rational product
     rng = np.random.default_rng(1)
