@@ -9,16 +9,18 @@
 //      (m, n, k) bucket table the autotuner fills, writes the exclusive prefix sum of
 //      tile counts (int64) and resets the queue head -- with kernel stores only, so
 //      the sequence can be captured in a hipGraph and replayed;
-//   2. batched_wave_kernel: every wavefront is an independent worker (no barriers,
-//      no LDS).  It takes a chunk of consecutive work items with one atomicAdd
-//      (the next chunk's atomic is in flight while the current chunk runs), finds
-//      each item's problem with a 64-way search of the prefix sums (one load per lane
-//      per level), and computes one C tile of 16x16 FP32 or FP64 MFMAs with its
-//      operand fragments loaded straight from the caller's column-major matrices
-//      into registers, one k stage ahead of the MFMAs.  Resident waves hide each
-//      other's memory latency; the grid is exactly the resident occupancy.
-// Tile kinds (one wavefront's C tile): 0 = 16x16, 1 = 32x32, 2 = 64x32 (rows x
-// columns), 3 = 32x64.
+//   2. batched_wg_kernel: persistent workgroups of 2 x 2 wavefronts.  A workgroup
+//      takes a chunk of consecutive work items with one atomicAdd (the next chunk's
+//      atomic is in flight while the current chunk runs), finds each item's problem
+//      with a 64-way search of the prefix sums (one load per lane per level), and
+//      computes one C tile with 16x16 FP32 or FP64 MFMAs.  The operands' k slices go
+//      from the caller's column-major matrices straight into LDS with buffer-to-LDS
+//      DMA (one 1 KiB instruction per 16 rows/columns of a slice, split over the four
+//      waves), through a three-slot ring that keeps two slices in flight while the
+//      MFMAs consume the third; one LDS-only barrier per slice.  The grid is exactly
+//      the resident occupancy.
+// Tile kinds (the workgroup's C tile, rows x columns): 0 = 32x32, 1 = 64x64,
+// 2 = 128x64, 3 = 64x128.
 #include "coral_common.hpp"
 #include "mfma.hpp"
 
@@ -43,20 +45,22 @@ __host__ __device__ inline int bucket_of(int x) {
   return b;
 }
 constexpr int kKinds = 4;
-// Problem p belongs to queue p % kXcd; the waves of workgroup w take items from queue
-// w % kXcd first.  Workgroups are dispatched to the chip's 8 XCDs round-robin, so a
-// problem's tiles run on one XCD and its operands are fetched into that XCD's L2
-// once instead of into all eight.  A wave whose queue is exhausted takes items from
-// the other queues.
+// Problem p belongs to queue p % kXcd; workgroup w takes items from queue w % kXcd
+// first.  Workgroups are dispatched to the chip's 8 XCDs round-robin, so a problem's
+// tiles run on one XCD and its operands are fetched into that XCD's L2 once instead
+// of into all eight.  A workgroup whose queue is exhausted takes items from the
+// other queues.
 constexpr int kXcd = 8;
 constexpr int kHeadStride = 16;  // 128 bytes between queue heads
-__host__ __device__ inline int kind_rows(int kind) { return kind == 0 ? 16 : (kind == 2 ? 64 : 32); }
-__host__ __device__ inline int kind_cols(int kind) { return kind == 0 ? 16 : (kind == 3 ? 64 : 32); }
-// Without a tuned entry: 16x16 tiles for problems within 16x16, 64x32 / 32x64 wave
-// tiles once both sides reach 64 (more MFMAs per fragment load), 32x32 otherwise.
+__host__ __device__ inline int kind_rows(int kind) { return kind == 0 ? 32 : (kind == 2 ? 128 : 64); }
+__host__ __device__ inline int kind_cols(int kind) { return kind == 0 ? 32 : (kind == 3 ? 128 : 64); }
+// Without a tuned entry: 32x32 tiles for problems within 32x32, 128x64 / 64x128 once
+// the long side reaches 128 and the short one 64 (fewer operand bytes per MFMA),
+// 64x64 otherwise.
 __host__ __device__ inline int default_kind(int m, int n, int) {
-  if (m <= 16 && n <= 16) return 0;
-  if (m >= 64 && n >= 64) return m >= n ? 2 : 3;
+  if (m <= 32 && n <= 32) return 0;
+  if (m >= 128 && n >= 64 && m >= n) return 2;
+  if (n >= 128 && m >= 64) return 3;
   return 1;
 }
 
@@ -152,10 +156,7 @@ __global__ __launch_bounds__(1024) void batched_setup_kernel(BatchArgs g) {
 
 // Work items are taken kChunk at a time (consecutive items: usually the tiles of one
 // problem), which keeps the queue head's atomics far below the L2's rate.
-constexpr int kChunk = 4;
-// k slices (of TR::KF each) per pipeline stage: two (32 k values for FP32, 16 for
-// FP64) measured faster than one for both precisions on the 10,000-problem workload
-constexpr int kStageSlices = 2;
+constexpr int kChunk = 2;
 
 // lane 0's value as a wave-uniform (scalar register) value.  Values the wave loads
 // from the descriptor arrays are the same in every lane, but the compiler cannot
@@ -197,93 +198,117 @@ __device__ __forceinline__ int64_t find_problem(const int64_t *__restrict__ off,
   return lo;
 }
 
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-
 // Raw buffer descriptor over [base, base + bytes) (bytes clamped to [0, 2^32 - 1]):
-// loads at or past the end return zeros (the gfx9-family untyped-buffer format word).
+// loads of dwords at or past the end return zeros (the gfx9-family untyped-buffer
+// format word).
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const char *base, int64_t bytes) {
   const int64_t nb = bytes <= 0 ? 0 : (bytes > (int64_t)0xFFFFFFFF ? (int64_t)0xFFFFFFFF : bytes);
   return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)(uint32_t)nb, 0x00020000);
 }
 
-// One element (4 or 8 bytes) at byte offset voff of a buffer, as raw bits.
-template <typename T>
-__device__ __forceinline__ T buf_elem(__amdgpu_buffer_rsrc_t r, int32_t voff) {
-  if constexpr (sizeof(T) == 4) {
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
+typedef __attribute__((address_space(3))) void *lds_ptr;
+
+// Buffer-to-LDS DMA of 16 bytes per lane (a non-template wrapper: the builtin's
+// operand checks then run once, outside any template instantiation).
+__device__ __forceinline__ void dma16(__amdgpu_buffer_rsrc_t r, char *dst, int32_t voff) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_ptr)dst, 16, voff, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Operand staging.  A k slice of KF values of one operand row (of op(A)) or column
+// (of op(B)) is 64 bytes for both precisions (16 floats, 8 doubles).  A tile's slice
+// of R rows/columns is R * 64 bytes, moved global -> LDS by R / 16 buffer-to-LDS DMA
+// instructions of 1 KiB (64 lanes x 16 bytes; lane l lands at the instruction's LDS
+// base + 16 l, and each lane chooses which 16 source bytes it fetches).  Two LDS
+// layouts, one per operand storage order:
+//
+//  !KC (the operand's rows are contiguous: element (r, k) at r + k ld): the slice is
+//      KF columns of R elements, column-major (quad L = k * R * es / 16 + r * es / 16),
+//      quad L stored at LDS quad position (L & ~15) | ((L & 15) ^ h(k)) with
+//      h(k) = es * ((k / VE) % (16 / es)): within every 256 bytes (all 64 banks) the
+//      16 / es column groups of one fragment read (lanes of one k offset t read
+//      column VE * grp + t, rows 16 i .. 16 i + 15) land on disjoint banks.  A fragment
+//      read is VE loads of one element.
+//  KC (k contiguous: element (r, k) at r ld + k): instruction j carries rows
+//      16 j .. 16 j + 15, each as four 16-byte quads; row c's quad q sits at LDS
+//      position 4 c + (q ^ ((c / 4) & 3)) (x 16 bytes), so the 16 lanes of one
+//      fragment read (16 rows, the same quad) cover every bank pair once.  A fragment
+//      read is one 16-byte load.
+//
+// Every DMA goes through a raw buffer descriptor rebased at the slice, so lane
+// offsets are 32-bit and bytes past the operand's last element read as zero (rows
+// past m / columns past n inside the operand only feed results that are not
+// stored).  The buffer unit range-checks every dword of a multi-dword load on its own
+// (a quad that straddles the operand's end keeps its dwords before the end), so the
+// last k slice of a tile, which can run past K, needs no other care for !KC operands
+// (columns past K lie past the operand's end); for KC operands the values past K
+// that lie inside the operand are zeroed after the fragment read (tail_mask).
+// ---------------------------------------------------------------------------
+constexpr int kBlock = 1024;              // LDS bytes per DMA instruction
+constexpr int kNS = 3;                    // LDS ring: slices in flight while one is used
+constexpr int kMaxRows = 192;             // BM + BN of the largest kind
+constexpr int kSlot = (kMaxRows / 16) * kBlock;
+
+// Byte offset (32-bit, from the slice base at (r0, k0)) of quad `chunk` of DMA
+// instruction j for operand storage KC / !KC, rows R.
+template <int ES, int VE>
+__device__ __forceinline__ int nkc_swizzle(int k) {
+  return ES * ((k / VE) % (16 / ES));
+}
+
+template <bool KC, int R, int ES, int VE>
+__device__ __forceinline__ int32_t quad_offset(int j, int chunk, int64_t ld) {
+  if constexpr (KC) {
+    const int c = chunk >> 2;                     // row within the instruction's 16
+    const int q = (chunk & 3) ^ ((c >> 2) & 3);   // quad held at this LDS position
+    return (int32_t)(((int64_t)c * ld) * ES + q * 16);  // relative to row 16 j
   } else {
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+    constexpr int QC = R * ES / 16;               // quads per slice column
+    const int pos = 64 * j + chunk;               // LDS quad position
+    const int u = pos >> 4;                       // 256-byte unit
+    // the column group, hence the swizzle, is fixed per unit: 16 u / (VE QC) = k / VE
+    const int h = ES * (((16 * u) / (VE * QC)) % (16 / ES));
+    const int L = (pos & ~15) | ((pos & 15) ^ h);  // natural quad index held here
+    const int k = L / QC, r = (L % QC) * (16 / ES);
+    return (int32_t)(((int64_t)r + (int64_t)k * ld) * ES);
   }
 }
 
-// The operand fragments of one wave: F 16-row (A) or 16-column (B) fragments starting
-// at operand row r0 (for B: column), streamed in k.  Element (r, k) of the operand is
-// at byte r * ld * es + k * es when k is contiguous (KC) and r * es + k * ld * es
-// otherwise.  Loads go through raw buffer descriptors rebased at each k slice, so
-// per-lane offsets are 32-bit and fixed for the whole tile, and everything past the
-// operand's last element reads as zero: rows/columns past the operand and, for !KC,
-// k past K need no clamping.  (Rows past m or columns past n inside the operand only
-// feed results that are not stored.)  For KC the k values past K inside the operand
-// are zeroed where the fragment is used (tail_mask).
-//
-// Lane l holds row r0 + 16 f + l % 16 and k values VE (l / 16) + t, t < VE, of
-// fragment f (the MFMA fragment order of mfma.hpp).
-template <class TR, bool KC, int F>
-struct OperandStream {
+// Issues DMA instruction j (rows / columns 16 j .. 16 j + 15) of one k slice (k0) of
+// one operand into that operand's LDS slice region.
+template <bool KC, int R, int ES, int VE>
+__device__ __forceinline__ void dma_one(const char *base, int64_t extent, int64_t ld, int64_t k0, char *region,
+                                        int j, int lane) {
+  // descriptor for this instruction: the slice (!KC), or its 16 rows (KC)
+  const int64_t off = KC ? ((int64_t)16 * j * ld + k0) * ES : k0 * ld * ES;
+  const __amdgpu_buffer_rsrc_t r = make_rsrc(base + off, extent - off);
+  dma16(r, region + kBlock * j, quad_offset<KC, R, ES, VE>(j, lane, ld));
+}
+
+// The fragment vector (VE consecutive k values at VE * grp) of operand row rho (a
+// multiple of 16 plus li) from one LDS slice region.
+template <class TR, bool KC, int R>
+__device__ __forceinline__ v4u read_frag(const char *region, int rho, int li, int grp) {
   using T = typename TR::T;
-  static constexpr int VE = TR::VE, KF = TR::KF, ES = (int)sizeof(T);
-  const char *base;  // byte address of (r0, k = 0)
-  int64_t extent;    // bytes from base to the end of the operand (<= 0: nothing left)
-  int64_t ld;        // elements
-  int32_t voff[KC ? 1 : VE];
-
-  __device__ __forceinline__ void init(const void *p, int64_t rows, int64_t K, int64_t ld_, int64_t r0, int lane) {
-    ld = ld_;
-    const int li = lane & 15, grp = lane >> 4;
-    // KC: rows of length K, ld apart; !KC: columns of length rows, ld apart
-    const int64_t total = KC ? (rows - 1) * ld + K : (K - 1) * ld + rows;
-    const int64_t start = KC ? r0 * ld : r0;
-    base = reinterpret_cast<const char *>(p) + start * ES;
-    extent = (total - start) * ES;
-    if constexpr (KC) {
-      voff[0] = (int32_t)(((int64_t)li * ld + VE * grp) * ES);
-    } else {
+  constexpr int VE = TR::VE, ES = (int)sizeof(T);
+  if constexpr (KC) {
+    const int pos = 4 * li + (grp ^ ((li >> 2) & 3));
+    return *reinterpret_cast<const v4u *>(region + 1024 * (rho >> 4) + 16 * pos);
+  } else {
+    T v[VE];
 #pragma unroll
-      for (int t = 0; t < VE; ++t) voff[t] = (int32_t)((li + (int64_t)(VE * grp + t) * ld) * ES);
+    for (int t = 0; t < VE; ++t) {
+      const int k = VE * grp + t;
+      const int b = (k * R + rho) * ES;  // natural byte offset in the column-major slice
+      const int L = b >> 4;
+      const int pos = (L & ~15) | ((L & 15) ^ nkc_swizzle<ES, VE>(k));
+      v[t] = *reinterpret_cast<const T *>(region + 16 * pos + (b & 15));
     }
+    v4u out;
+    __builtin_memcpy(&out, &v[0], 16);
+    return out;
   }
-
-  // the F fragments of the k slice starting at k0 (full: k0 + KF <= K)
-  __device__ __forceinline__ void load(v4u (&out)[F], int64_t k0, bool full) const {
-    if constexpr (KC) {
-#pragma unroll
-      for (int f = 0; f < F; ++f) {
-        // one descriptor per fragment: its 16 rows start 16 f rows further on
-        const int64_t off = ((int64_t)16 * f * ld + k0) * ES;
-        const __amdgpu_buffer_rsrc_t r = make_rsrc(base + off, extent - off);
-        if (full) {
-          out[f] = __builtin_bit_cast(v4u, __builtin_amdgcn_raw_buffer_load_b128(r, voff[0], 0, 0));
-        } else {
-          T v[VE];
-#pragma unroll
-          for (int t = 0; t < VE; ++t) v[t] = buf_elem<T>(r, voff[0] + t * ES);
-          __builtin_memcpy(&out[f], &v[0], 16);
-        }
-      }
-    } else {
-      (void)full;
-      const int64_t off = k0 * ld * ES;
-      const __amdgpu_buffer_rsrc_t r = make_rsrc(base + off, extent - off);
-#pragma unroll
-      for (int f = 0; f < F; ++f) {
-        T v[VE];
-#pragma unroll
-        for (int t = 0; t < VE; ++t) v[t] = buf_elem<T>(r, voff[t] + 16 * f * ES);
-        __builtin_memcpy(&out[f], &v[0], 16);
-      }
-    }
-  }
-};
+}
 
 // Zero the values of a fragment vector whose k (kb + t) is at or past K.
 template <class TR>
@@ -298,77 +323,127 @@ __device__ __forceinline__ v4u tail_mask(v4u f, int64_t kb, int64_t K) {
   return f;
 }
 
+// Workgroup barrier that orders LDS only: this wave's LDS reads and writes (and scalar
+// loads) are complete, then s_barrier.  __syncthreads() would also wait for every
+// vector memory operation in flight (its release fence), i.e. for the DMAs of the
+// slices still being prefetched.  The LDS DMA of a slice is waited for explicitly
+// (wait_vmcnt) before the barrier that publishes it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Waits until at most N vector memory operations of this wave are in flight.
+template <int N> __device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N <= 3, "vmcnt immediates used here");
+  if constexpr (N == 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  else if constexpr (N == 1) asm volatile("s_waitcnt vmcnt(1)" ::: "memory");
+  else if constexpr (N == 2) asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
+  else asm volatile("s_waitcnt vmcnt(3)" ::: "memory");
+}
+
 struct Problem {
   int64_t m, n, K, lda, ldb, ldc;
   const void *a, *b;
   void *c;
 };
 
-// One wavefront computes the FM x FN block of 16x16 tiles at tile index t of problem
-// P: C := alpha * op(A) op(B) + beta * C on its valid part.
-template <class TR, bool AKC, bool BKC, int FM, int FN>
-__device__ __forceinline__ void wave_tile(const BatchArgs &g, const Problem &P, int64_t t, int lane) {
+// One workgroup (2 x 2 waves) computes the BM x BN tile t of problem P:
+// C := alpha * op(A) op(B) + beta * C on its valid part.  The k slices stream through
+// a three-slot LDS ring (three arrays), two slices ahead of the MFMAs; one barrier
+// per slice.
+template <class TR, bool AKC, bool BKC, int BM, int BN>
+__device__ __forceinline__ void wg_tile(const BatchArgs &g, const Problem &P, int64_t t, char *ring0, char *ring1,
+                                        char *ring2) {
   using T = typename TR::T;
   using acc_t = typename TR::acc_t;
-  constexpr int KF = TR::KF, VE = TR::VE, KS = kStageSlices;
-  constexpr int64_t KST = (int64_t)KF * KS;  // k values per stage
-  const int li = lane & 15, grp = lane >> 4;
-  const int64_t tiles_m = (P.m + 16 * FM - 1) / (16 * FM);
-  const int64_t r0 = (t % tiles_m) * (16 * FM), c0 = (t / tiles_m) * (16 * FN);
-  // op(A) rows are k-contiguous for Transpose (AKC); op(B) columns for NoTranspose (BKC)
-  OperandStream<TR, AKC, FM> sa;
-  OperandStream<TR, BKC, FN> sb;
+  constexpr int KF = TR::KF, VE = TR::VE, ES = (int)sizeof(T);
+  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+  constexpr int NIA = BM / 16, NIB = BN / 16, NI = NIA + NIB;
+  static_assert(NI % 4 == 0 && NI <= kMaxRows / 16, "DMA instructions split evenly over the waves");
+  constexpr int D = NI / 4;  // DMA instructions per wave per slice
+  constexpr int A_BYTES = NIA * kBlock;
+  const int lane = threadIdx.x & 63;
+  const int wave = uniform32((int)(threadIdx.x >> 6));
+  const int wm = wave & 1, wn = wave >> 1, li = lane & 15, grp = lane >> 4;
+  const int64_t tiles_m = (P.m + BM - 1) / BM;
+  const int64_t r0 = (t % tiles_m) * BM, c0 = (t / tiles_m) * BN;
   const int64_t K = g.alpha == 0.0 ? 0 : P.K;  // alpha == 0: A and B are not read
-  if (K > 0) {
-    sa.init(P.a, P.m, K, P.lda, r0, lane);
-    sb.init(P.b, P.n, K, P.ldb, c0, lane);
-  }
+  const int64_t nst = (K + KF - 1) / KF;
+  // this wave's 16 x 16 fragments that hold rows < m (columns < n): the others are
+  // not computed
+  const int fmv = (int)imin64(FM, imax64(0, (P.m - r0 - wm * WM + 15) / 16));
+  const int fnv = (int)imin64(FN, imax64(0, (P.n - c0 - wn * WN + 15) / 16));
+
+  // operand bases at (r0, k = 0) and bytes to the operand's end
+  // KC: rows of length K, ld apart; !KC: columns of length rows, ld apart
+  const int64_t a_total = AKC ? (P.m - 1) * P.lda + K : (K - 1) * P.lda + P.m;
+  const int64_t a_start = AKC ? r0 * P.lda : r0;
+  const char *a_base = reinterpret_cast<const char *>(P.a) + a_start * ES;
+  const int64_t a_ext = (a_total - a_start) * ES;
+  const int64_t b_total = BKC ? (P.n - 1) * P.ldb + K : (K - 1) * P.ldb + P.n;
+  const int64_t b_start = BKC ? c0 * P.ldb : c0;
+  const char *b_base = reinterpret_cast<const char *>(P.b) + b_start * ES;
+  const int64_t b_ext = (b_total - b_start) * ES;
+
+  // this wave's DMA instructions of slice s_ into slot: instruction g of the
+  // combined (A then B) list of NI goes to wave g % 4
+  auto issue = [&](char *slot, int64_t s_) {
+    const int64_t k0_ = s_ * KF;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const int gi = wave + 4 * i;
+      if (gi < NIA) dma_one<AKC, BM, ES, VE>(a_base, a_ext, P.lda, k0_, slot, gi, lane);
+      else dma_one<BKC, BN, ES, VE>(b_base, b_ext, P.ldb, k0_, slot + A_BYTES, gi - NIA, lane);
+    }
+  };
+
   acc_t acc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
-  const int64_t nst = (K + KST - 1) / KST;
-  v4u fa0[KS][FM], fb0[KS][FN], fa1[KS][FM], fb1[KS][FN];
-  auto load = [&](int64_t st, v4u(&fa)[KS][FM], v4u(&fb)[KS][FN]) {
+
+  // slice s uses slot s % 3 and, after the barrier, refills slot (s + 2) % 3 (the one
+  // slice s - 1 used) with slice s + 2
+  auto step = [&](const char *cur, char *refill, int64_t s_) {
+    // this wave's DMAs of slice s_ are done once at most those of slice s_ + 1 are
+    // pending (vector memory operations of one kind complete in issue order)
+    if (s_ + 1 < nst) wait_vmcnt<D>();
+    else wait_vmcnt<0>();
+    lds_barrier();  // every wave's slice s_ is in LDS; every wave is done with slice s_ - 1
+    if (s_ + 2 < nst) issue(refill, s_ + 2);
+    const int64_t k0 = s_ * KF;
+    v4u fa[FM], fb[FN];
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      const int64_t k0 = st * KST + s * KF;
-      const bool full = k0 + KF <= K;
-      sa.load(fa[s], k0, full);
-      sb.load(fb[s], k0, full);
-    }
-  };
-  auto compute = [&](int64_t st, v4u(&fa)[KS][FM], v4u(&fb)[KS][FN]) {
+    for (int i = 0; i < FM; ++i) fa[i] = read_frag<TR, AKC, BM>(cur, wm * WM + 16 * i + li, li, grp);
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      const int64_t k0 = st * KST + s * KF;
-      if (s > 0 && k0 >= K) break;  // slices past K: nothing to add
-      if (k0 + KF > K) {            // the last slice: zero the k-contiguous values past K
-        const int64_t kb = k0 + VE * grp;
-        if constexpr (AKC) {
+    for (int j = 0; j < FN; ++j) fb[j] = read_frag<TR, BKC, BN>(cur + A_BYTES, wn * WN + 16 * j + li, li, grp);
+    if (k0 + KF > K) {  // the last slice: zero the k-contiguous values past K
+      const int64_t kb = k0 + VE * grp;
+      if constexpr (AKC) {
 #pragma unroll
-          for (int i = 0; i < FM; ++i) fa[s][i] = tail_mask<TR>(fa[s][i], kb, K);
-        }
-        if constexpr (BKC) {
-#pragma unroll
-          for (int j = 0; j < FN; ++j) fb[s][j] = tail_mask<TR>(fb[s][j], kb, K);
-        }
+        for (int i = 0; i < FM; ++i) fa[i] = tail_mask<TR>(fa[i], kb, K);
       }
+      if constexpr (BKC) {
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) TR::mma(acc[i][j], &fa[s][i], &fb[s][j]);
+        for (int j = 0; j < FN; ++j) fb[j] = tail_mask<TR>(fb[j], kb, K);
+      }
     }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        if (i < fmv && j < fnv) TR::mma(acc[i][j], &fa[i], &fb[j]);
   };
-  if (nst > 0) load(0, fa0, fb0);
-  for (int64_t st = 0; st < nst; st += 2) {
-    if (st + 1 < nst) load(st + 1, fa1, fb1);
-    compute(st, fa0, fb0);
-    if (st + 1 < nst) {
-      if (st + 2 < nst) load(st + 2, fa0, fb0);
-      compute(st + 1, fa1, fb1);
-    }
+
+  // the previous tile's C stores (which share the vector memory counter with the
+  // DMAs, in no fixed order relative to them) have completed
+  wait_vmcnt<0>();
+  if (nst > 0) issue(ring0, 0);
+  if (nst > 1) issue(ring1, 1);
+  // unrolled by the ring size so that every LDS access names its slot's own array
+  for (int64_t s = 0; s < nst; s += kNS) {
+    step(ring0, ring2, s);
+    if (s + 1 < nst) step(ring1, ring0, s + 1);
+    if (s + 2 < nst) step(ring2, ring1, s + 2);
   }
   const T al = (T)g.alpha, be = (T)g.beta;
   __attribute__((address_space(1))) T *C = (__attribute__((address_space(1))) T *)reinterpret_cast<T *>(P.c);
@@ -376,10 +451,10 @@ __device__ __forceinline__ void wave_tile(const BatchArgs &g, const Problem &P, 
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
-      const int64_t col = c0 + 16 * j + li;
+      const int64_t col = c0 + wn * WN + 16 * j + li;
       // 16x16 accumulator: lane l holds column l % 16; register r holds row
       // 4 * (l / 16) + r (f32 MFMA) or l / 16 + 4 * r (f64 MFMA)
-      const int64_t row0 = r0 + 16 * i + (TR::ROWS_INTERLEAVED ? grp : 4 * grp);
+      const int64_t row0 = r0 + wm * WM + 16 * i + (TR::ROWS_INTERLEAVED ? grp : 4 * grp);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t row = row0 + (TR::ROWS_INTERLEAVED ? 4 * r : r);
@@ -391,11 +466,18 @@ __device__ __forceinline__ void wave_tile(const BatchArgs &g, const Problem &P, 
         }
       }
     }
+  lds_barrier();  // the ring's slots are free for the next tile
 }
 
 template <class TR, bool AKC, bool BKC>
-__global__ __launch_bounds__(256) void batched_wave_kernel(BatchArgs g) {
-  const int lane = threadIdx.x & 63;
+__global__ __launch_bounds__(256) void batched_wg_kernel(BatchArgs g) {
+  // the LDS ring: three separate arrays, so that the compiler can tell a read of one
+  // slot from a DMA into another and does not wait on that DMA
+  __shared__ __attribute__((aligned(16))) char ring0[kSlot];
+  __shared__ __attribute__((aligned(16))) char ring1[kSlot];
+  __shared__ __attribute__((aligned(16))) char ring2[kSlot];
+  __shared__ unsigned long long s_chunk;
+  const int t = threadIdx.x, lane = t & 63;
   const int home = blockIdx.x % kXcd;
   for (int qi = 0; qi < kXcd; ++qi) {  // the home queue, then the others
     const int q = (home + qi) % kXcd;
@@ -403,14 +485,14 @@ __global__ __launch_bounds__(256) void batched_wave_kernel(BatchArgs g) {
     if (nq <= 0) continue;
     const int64_t total = sload(g.offsets, g.batch + q);
     unsigned long long *head = g.head + q * kHeadStride;
-    unsigned long long chunk = 0;
-    if (lane == 0) chunk = atomicAdd(head, 1ull);
-    chunk = uniform64(chunk);
+    if (t == 0) s_chunk = atomicAdd(head, 1ull);
+    __syncthreads();
     for (;;) {
+      const unsigned long long chunk = uniform64(s_chunk);
+      lds_barrier();  // every thread has read s_chunk
       const int64_t item0 = (int64_t)chunk * kChunk;
-      if (item0 >= total) break;  // uniform over the wave: this queue is exhausted
-      unsigned long long next = 0;
-      if (lane == 0) next = atomicAdd(head, 1ull);  // in flight while this chunk runs
+      if (item0 >= total) break;  // uniform over the workgroup: this queue is exhausted
+      if (t == 0) s_chunk = atomicAdd(head, 1ull);  // the next chunk, in flight meanwhile
       int64_t j = (int64_t)uniform64((uint64_t)find_problem(g.offsets, q, nq, item0, lane));
       int64_t j_end = j + 1 < nq ? sload(g.offsets, q + kXcd * (j + 1)) : total;
       for (int c = 0; c < kChunk; ++c) {
@@ -431,15 +513,15 @@ __global__ __launch_bounds__(256) void batched_wave_kernel(BatchArgs g) {
         P.a = sload(g.a, p);
         P.b = sload(g.b, p);
         P.c = sload(g.c, p);
-        const int64_t t = item - sload(g.offsets, p);
+        const int64_t tt = item - sload(g.offsets, p);
         switch (sload(g.kinds, p)) {
-          case 0: wave_tile<TR, AKC, BKC, 1, 1>(g, P, t, lane); break;
-          case 1: wave_tile<TR, AKC, BKC, 2, 2>(g, P, t, lane); break;
-          case 2: wave_tile<TR, AKC, BKC, 4, 2>(g, P, t, lane); break;
-          default: wave_tile<TR, AKC, BKC, 2, 4>(g, P, t, lane); break;
+          case 0: wg_tile<TR, AKC, BKC, 32, 32>(g, P, tt, ring0, ring1, ring2); break;
+          case 1: wg_tile<TR, AKC, BKC, 64, 64>(g, P, tt, ring0, ring1, ring2); break;
+          case 2: wg_tile<TR, AKC, BKC, 128, 64>(g, P, tt, ring0, ring1, ring2); break;
+          default: wg_tile<TR, AKC, BKC, 64, 128>(g, P, tt, ring0, ring1, ring2); break;
         }
       }
-      chunk = uniform64(next);
+      lds_barrier();  // thread 0's next chunk is in s_chunk
     }
   }
 }
@@ -518,10 +600,10 @@ static int batched_impl(int dt, int32_t op_a, int32_t op_b, int64_t batch, const
   CORAL_LAUNCH_CHECK();
   // op(A) rows k-contiguous (Transpose) / op(B) columns k-contiguous (NoTranspose)
   const int layout = (g.op_a != CORAL_NO_TRANSPOSE ? 2 : 0) + (g.op_b == CORAL_NO_TRANSPOSE ? 1 : 0);
-  void (*kern)(BatchArgs) = layout == 0 ? batched_wave_kernel<TR, false, false>
-                          : layout == 1 ? batched_wave_kernel<TR, false, true>
-                          : layout == 2 ? batched_wave_kernel<TR, true, false>
-                                        : batched_wave_kernel<TR, true, true>;
+  void (*kern)(BatchArgs) = layout == 0 ? batched_wg_kernel<TR, false, false>
+                          : layout == 1 ? batched_wg_kernel<TR, false, true>
+                          : layout == 2 ? batched_wg_kernel<TR, true, false>
+                                        : batched_wg_kernel<TR, true, true>;
   // persistent: exactly as many workgroups as can be resident at once
   static int per_cu[2][4] = {};
   if (per_cu[dt][layout] == 0) {
