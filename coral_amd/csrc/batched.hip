@@ -10,10 +10,10 @@
 //      tile counts (int64) and resets the queue head -- with kernel stores only, so
 //      the sequence can be captured in a hipGraph and replayed;
 //   2. batched_wg_kernel: persistent workgroups of 2 x 2 wavefronts.  A workgroup
-//      takes a chunk of consecutive work items with one atomicAdd (the next chunk's
-//      atomic is in flight while the current chunk runs), finds each item's problem
-//      with a 64-way search of the prefix sums (one load per lane per level), and
-//      computes one C tile with 16x16 FP32 or FP64 MFMAs.  The operands' k slices go
+//      takes one work item (a C tile) at a time with an atomicAdd on its queue head,
+//      issued a tile ahead; it finds the item's problem with a 64-way search of the
+//      prefix sums (the next 64 problems first) between the current tile's last MFMA
+//      and its stores, and computes the tile with 16x16 FP32 or FP64 MFMAs.  The operands' k slices go
 //      from the caller's column-major matrices straight into LDS with buffer-to-LDS
 //      DMA (one 1 KiB instruction per 16 rows/columns of a slice, split over the four
 //      waves), through a three-slot ring that keeps two slices in flight while the
@@ -154,10 +154,6 @@ __global__ __launch_bounds__(1024) void batched_setup_kernel(BatchArgs g) {
   }
 }
 
-// Work items are taken kChunk at a time (consecutive items: usually the tiles of one
-// problem), which keeps the queue head's atomics far below the L2's rate.
-constexpr int kChunk = 2;
-
 // lane 0's value as a wave-uniform (scalar register) value.  Values the wave loads
 // from the descriptor arrays are the same in every lane, but the compiler cannot
 // prove it (the arrays may alias C), so they are made uniform explicitly; loop
@@ -180,12 +176,12 @@ template <typename T> __device__ __forceinline__ T sload(const T *p, int64_t i) 
 }
 
 // Queue q holds problems q, q + kXcd, q + 2 kXcd, ...: the j-th of them has prefix sum
-// off[q + kXcd j].  Returns the largest j < nq with off[q + kXcd j] <= item (item is
-// below the queue's total): a 64-way search, each level one load per lane and a
-// ballot.  That problem has tiles, so problems with none are passed over.
-__device__ __forceinline__ int64_t find_problem(const int64_t *__restrict__ off, int q, int64_t nq, int64_t item,
-                                                int lane) {
-  int64_t lo = 0, hi = nq;
+// off[q + kXcd j].  Returns the largest j in [lo, hi) with off[q + kXcd j] <= item
+// (off[q + kXcd lo] <= item < the queue's total): a 64-way search, each level one load
+// per lane and a ballot.  That problem has tiles, so problems with none are passed
+// over.
+__device__ __forceinline__ int64_t find_problem(const int64_t *__restrict__ off, int q, int64_t lo, int64_t hi,
+                                                int64_t item, int lane) {
   while (hi - lo > 1) {
     const int64_t step = (hi - lo + 63) >> 6;
     const int64_t idx = lo + (int64_t)lane * step;
@@ -196,6 +192,62 @@ __device__ __forceinline__ int64_t find_problem(const int64_t *__restrict__ off,
     hi = lo + step < hi ? lo + step : hi;
   }
   return lo;
+}
+
+__device__ __forceinline__ int64_t readlane64(int64_t v, int l) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)(uint64_t)v, l);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)((uint64_t)v >> 32), l);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+struct Problem {
+  int64_t m, n, K, lda, ldb, ldc;
+  const void *a, *b;
+  void *c;
+};
+
+// A work item's place: problem j of its queue (problem index q + kXcd j), whose
+// tiles are the items [start, end), with that problem's descriptor and tile kind.
+struct Cursor {
+  int64_t j, start, end;
+  Problem P;
+  int kind;
+};
+
+// The cursor of `item` of queue q, given the cursor `from` of an earlier item (items
+// only grow, so the problem is from's or a later one; from.j = -1, from.end = 0 for
+// none).  The next 64 problems are tried first (one load per lane), then the rest
+// of the queue is searched.
+__device__ __forceinline__ Cursor locate(const BatchArgs &g, int q, int64_t nq, int64_t total, int64_t item,
+                                         const Cursor &from, int lane) {
+  Cursor c = from;
+  if (item < from.end) return c;
+  const int64_t lo = from.j + 1;  // off[q + kXcd lo] = from.end <= item
+  const int64_t idx = lo + lane;
+  const int64_t v = idx < nq ? g.offsets[q + kXcd * idx] : INT64_MAX;
+  const uint64_t mask = __ballot(v <= item);
+  const int top = 63 - __builtin_clzll(mask);
+  if (top < 63 || lo + 64 >= nq) {
+    c.j = lo + top;
+    c.start = readlane64(v, top);
+    c.end = lo + top + 1 < nq ? readlane64(v, top + 1) : total;
+  } else {
+    c.j = (int64_t)uniform64((uint64_t)find_problem(g.offsets, q, lo + 63, nq, item, lane));
+    c.start = sload(g.offsets, q + kXcd * c.j);
+    c.end = c.j + 1 < nq ? sload(g.offsets, q + kXcd * (c.j + 1)) : total;
+  }
+  const int64_t p = q + kXcd * c.j;
+  c.P.m = sload(g.m, p);
+  c.P.n = sload(g.n, p);
+  c.P.K = sload(g.k, p);
+  c.P.lda = sload(g.lda, p);
+  c.P.ldb = sload(g.ldb, p);
+  c.P.ldc = sload(g.ldc, p);
+  c.P.a = sload(g.a, p);
+  c.P.b = sload(g.b, p);
+  c.P.c = sload(g.c, p);
+  c.kind = sload(g.kinds, p);
+  return c;
 }
 
 // Raw buffer descriptor over [base, base + bytes) (bytes clamped to [0, 2^32 - 1]):
@@ -330,6 +382,10 @@ __device__ __forceinline__ v4u tail_mask(v4u f, int64_t kb, int64_t K) {
 // (wait_vmcnt) before the barrier that publishes it.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// s_waitcnt vmcnt(0) with the other counters not waited for (gfx9 encoding: vmcnt
+// bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+constexpr int kWaitVm0 = (0x7 << 4) | (0xF << 8);
+
 // Waits until at most N vector memory operations of this wave are in flight.
 template <int N> __device__ __forceinline__ void wait_vmcnt() {
   static_assert(N >= 0 && N <= 3, "vmcnt immediates used here");
@@ -339,19 +395,16 @@ template <int N> __device__ __forceinline__ void wait_vmcnt() {
   else asm volatile("s_waitcnt vmcnt(3)" ::: "memory");
 }
 
-struct Problem {
-  int64_t m, n, K, lda, ldb, ldc;
-  const void *a, *b;
-  void *c;
-};
-
 // One workgroup (2 x 2 waves) computes the BM x BN tile t of problem P:
 // C := alpha * op(A) op(B) + beta * C on its valid part.  The k slices stream through
 // a three-slot LDS ring (three arrays), two slices ahead of the MFMAs; one barrier
-// per slice.
-template <class TR, bool AKC, bool BKC, int BM, int BN>
+// per slice.  look() runs between the last MFMA and the stores of C (it finds the
+// next tile: its loads then wait for nothing but themselves).  The stores are not
+// waited for: the next tile's first DMAs go out behind them, and the first slice's
+// wait covers both.
+template <class TR, bool AKC, bool BKC, int BM, int BN, class Look>
 __device__ __forceinline__ void wg_tile(const BatchArgs &g, const Problem &P, int64_t t, char *ring0, char *ring1,
-                                        char *ring2) {
+                                        char *ring2, Look &look) {
   using T = typename TR::T;
   using acc_t = typename TR::acc_t;
   constexpr int KF = TR::KF, VE = TR::VE, ES = (int)sizeof(T);
@@ -431,12 +484,12 @@ __device__ __forceinline__ void wg_tile(const BatchArgs &g, const Problem &P, in
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        if (i < fmv && j < fnv) TR::mma(acc[i][j], &fa[i], &fb[j]);
+        if (i < fmv && j < fnv) TR::mma(acc[i][j], &fb[j], &fa[i]);  // C^T tile
   };
 
-  // the previous tile's C stores (which share the vector memory counter with the
-  // DMAs, in no fixed order relative to them) have completed
-  wait_vmcnt<0>();
+  // The vector memory counter also holds the previous tile's C stores and the queue
+  // atomic, all issued before these DMAs: a wait for "at most the DMAs of slice s + 1
+  // pending" still implies slice s is done, since DMAs complete in issue order.
   if (nst > 0) issue(ring0, 0);
   if (nst > 1) issue(ring1, 1);
   // unrolled by the ring size so that every LDS access names its slot's own array
@@ -445,38 +498,74 @@ __device__ __forceinline__ void wg_tile(const BatchArgs &g, const Problem &P, in
     if (s + 1 < nst) step(ring1, ring0, s + 1);
     if (s + 2 < nst) step(ring2, ring1, s + 2);
   }
+  look();
+  // The MFMAs compute the transposed tile (B fragment as the first operand), so
+  // lane l holds C row l % 16 and register r holds column 4 * (l / 16) + r (f32 MFMA)
+  // or l / 16 + 4 * r (f64 MFMA): a store instruction writes 16 consecutive rows of
+  // four columns, i.e. four contiguous 64- or 128-byte runs.  beta == 0 and beta != 0
+  // are separate paths: in the first no load of C exists whose destination registers
+  // a store would have to wait for (which, with stores in flight, would be a wait for
+  // all of them); in the second the four old values of a fragment are loaded
+  // before its stores.
   const T al = (T)g.alpha, be = (T)g.beta;
   __attribute__((address_space(1))) T *C = (__attribute__((address_space(1))) T *)reinterpret_cast<T *>(P.c);
+  const int64_t row_base = r0 + wm * WM + li;
+  const int64_t col_base = c0 + wn * WN + (TR::ROWS_INTERLEAVED ? grp : 4 * grp);
+  auto cidx = [&](int i, int j, int r, bool &in) {
+    const int64_t row = row_base + 16 * i;
+    const int64_t col = col_base + 16 * j + (TR::ROWS_INTERLEAVED ? 4 * r : r);
+    in = row < P.m && col < P.n;
+    return row + col * P.ldc;
+  };
+  if (be == T(0)) {
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int64_t col = c0 + wn * WN + 16 * j + li;
-      // 16x16 accumulator: lane l holds column l % 16; register r holds row
-      // 4 * (l / 16) + r (f32 MFMA) or l / 16 + 4 * r (f64 MFMA)
-      const int64_t row0 = r0 + wm * WM + 16 * i + (TR::ROWS_INTERLEAVED ? grp : 4 * grp);
+      for (int j = 0; j < FN; ++j) {
+        if (!(i < fmv && j < fnv)) continue;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t row = row0 + (TR::ROWS_INTERLEAVED ? 4 * r : r);
-        if (row < P.m && col < P.n) {
-          __attribute__((address_space(1))) T *cp = C + row + col * P.ldc;
-          T o = al * acc[i][j][r];
-          if (be != T(0)) o += be * (*cp);
-          *cp = o;
+        for (int r = 0; r < 4; ++r) {
+          bool in;
+          const int64_t x = cidx(i, j, r, in);
+          if (in) C[x] = al * acc[i][j][r];
         }
       }
-    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        if (!(i < fmv && j < fnv)) continue;
+        T old[4];
+        bool in[4];
+        int64_t x[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          x[r] = cidx(i, j, r, in[r]);
+          old[r] = in[r] ? C[x[r]] : T(0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          T o = al * acc[i][j][r];  // the single-problem routine's epilogue, operation for operation
+          o += be * old[r];
+          if (in[r]) C[x[r]] = o;
+        }
+      }
+    // an s_waitcnt the compiler sees: after the join no load of this branch is still
+    // pending in its bookkeeping, so the code that follows has no waits inserted for it
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+  }
   lds_barrier();  // the ring's slots are free for the next tile
 }
 
 template <class TR, bool AKC, bool BKC>
-__global__ __launch_bounds__(256) void batched_wg_kernel(BatchArgs g) {
+__global__ __launch_bounds__(256, 4) void batched_wg_kernel(BatchArgs g) {
   // the LDS ring: three separate arrays, so that the compiler can tell a read of one
   // slot from a DMA into another and does not wait on that DMA
   __shared__ __attribute__((aligned(16))) char ring0[kSlot];
   __shared__ __attribute__((aligned(16))) char ring1[kSlot];
   __shared__ __attribute__((aligned(16))) char ring2[kSlot];
-  __shared__ unsigned long long s_chunk;
+  __shared__ unsigned long long s_item;
   const int t = threadIdx.x, lane = t & 63;
   const int home = blockIdx.x % kXcd;
   for (int qi = 0; qi < kXcd; ++qi) {  // the home queue, then the others
@@ -485,43 +574,41 @@ __global__ __launch_bounds__(256) void batched_wg_kernel(BatchArgs g) {
     if (nq <= 0) continue;
     const int64_t total = sload(g.offsets, g.batch + q);
     unsigned long long *head = g.head + q * kHeadStride;
-    if (t == 0) s_chunk = atomicAdd(head, 1ull);
+    if (t == 0) s_item = atomicAdd(head, 1ull);
     __syncthreads();
+    int64_t item = (int64_t)uniform64(s_item);
+    lds_barrier();  // every thread has read s_item
+    if (item >= total) continue;  // uniform over the workgroup: this queue is exhausted
+    Cursor none;
+    none.j = -1;
+    none.start = none.end = 0;
+    Cursor cur = locate(g, q, nq, total, item, none, lane);
+    // thread 0: the queue atomic for the item after the next one to be read, issued
+    // once the search's loads are done; its value is published one tile later
+    unsigned long long ahead = 0;
+    if (t == 0) ahead = atomicAdd(head, 1ull);
     for (;;) {
-      const unsigned long long chunk = uniform64(s_chunk);
-      lds_barrier();  // every thread has read s_chunk
-      const int64_t item0 = (int64_t)chunk * kChunk;
-      if (item0 >= total) break;  // uniform over the workgroup: this queue is exhausted
-      if (t == 0) s_chunk = atomicAdd(head, 1ull);  // the next chunk, in flight meanwhile
-      int64_t j = (int64_t)uniform64((uint64_t)find_problem(g.offsets, q, nq, item0, lane));
-      int64_t j_end = j + 1 < nq ? sload(g.offsets, q + kXcd * (j + 1)) : total;
-      for (int c = 0; c < kChunk; ++c) {
-        const int64_t item = item0 + c;
-        if (item >= total) break;
-        while (item >= j_end) {  // the next problem of the queue with tiles
-          ++j;
-          j_end = j + 1 < nq ? sload(g.offsets, q + kXcd * (j + 1)) : total;
+      Cursor nx = cur;
+      int64_t next = total;
+      auto look = [&]() {
+        if (t == 0) s_item = ahead;
+        lds_barrier();  // s_item is published (and every read of its previous value is done)
+        next = (int64_t)uniform64(s_item);
+        if (next < total) {
+          nx = locate(g, q, nq, total, next, cur, lane);
+          if (t == 0) ahead = atomicAdd(head, 1ull);
         }
-        const int64_t p = q + kXcd * j;
-        Problem P;
-        P.m = sload(g.m, p);
-        P.n = sload(g.n, p);
-        P.K = sload(g.k, p);
-        P.lda = sload(g.lda, p);
-        P.ldb = sload(g.ldb, p);
-        P.ldc = sload(g.ldc, p);
-        P.a = sload(g.a, p);
-        P.b = sload(g.b, p);
-        P.c = sload(g.c, p);
-        const int64_t tt = item - sload(g.offsets, p);
-        switch (sload(g.kinds, p)) {
-          case 0: wg_tile<TR, AKC, BKC, 32, 32>(g, P, tt, ring0, ring1, ring2); break;
-          case 1: wg_tile<TR, AKC, BKC, 64, 64>(g, P, tt, ring0, ring1, ring2); break;
-          case 2: wg_tile<TR, AKC, BKC, 128, 64>(g, P, tt, ring0, ring1, ring2); break;
-          default: wg_tile<TR, AKC, BKC, 64, 128>(g, P, tt, ring0, ring1, ring2); break;
-        }
+      };
+      const int64_t tt = item - cur.start;
+      switch (cur.kind) {
+        case 0: wg_tile<TR, AKC, BKC, 32, 32>(g, cur.P, tt, ring0, ring1, ring2, look); break;
+        case 1: wg_tile<TR, AKC, BKC, 64, 64>(g, cur.P, tt, ring0, ring1, ring2, look); break;
+        case 2: wg_tile<TR, AKC, BKC, 128, 64>(g, cur.P, tt, ring0, ring1, ring2, look); break;
+        default: wg_tile<TR, AKC, BKC, 64, 128>(g, cur.P, tt, ring0, ring1, ring2, look); break;
       }
-      lds_barrier();  // thread 0's next chunk is in s_chunk
+      if (next >= total) break;  // uniform: this queue is exhausted
+      cur = nx;
+      item = next;
     }
   }
 }

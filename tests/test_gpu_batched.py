@@ -86,7 +86,11 @@ def check(batch, alpha, beta, eps):
         B = ref64.mat(b, br, bc, ldb) if k else np.zeros((1, 1))
         C = ref64.mat(c, m, n, ldc)
         num = np.linalg.norm(ref64.mat(o, m, n, ldc).astype(np.float64) - ref64.mat(ref, m, n, ldc))
-        den = max(k, 1) * eps * (abs(alpha) * np.linalg.norm(A) * np.linalg.norm(B) + abs(beta) * np.linalg.norm(C))
+        # gamma_{k+2}: k roundings in the dot products, one in the alpha scaling and one
+        # in the beta update (which the compiler may fuse either way: fma(alpha, acc,
+        # beta c) or fma(beta, c, alpha acc)); at k = 1 the k-only bound is exceeded by
+        # a correctly rounded fma.
+        den = (k + 2) * eps * (abs(alpha) * np.linalg.norm(A) * np.linalg.norm(B) + abs(beta) * np.linalg.norm(C))
         worst = max(worst, num / (den + 1e-300))
     return worst
 
