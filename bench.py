@@ -44,7 +44,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "SGEMM 8192^3 TFLOP/s BF16x3 vs FP32 MFMA at equal FP64-oracle error; GEMV GB/s"
-TRAFFIC_PROFILES = {f: os.path.join(ROOT, "profiles", f"gemm_{f}_8192_traffic.md") for f in ("bf16x3", "fp16x3")}
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "gemm_8192_headline_traffic.md")
 PEAK_BF16 = 2500.0  # dense BF16 MFMA TFLOP/s (MI355X)
 PEAK_FP32 = 157.3
 PEAK_FP64 = 78.6
@@ -96,13 +96,14 @@ def sample_error(A, B, C, rows, k):
             "max_abs_over_max_ref": float(np.abs(E).max() / np.abs(ref).max())}
 
 
-def profiled_traffic(path):
-    """HBM bytes per launch of a GEMM kernel from its committed PMC summary (a line
-    `traffic_bytes_per_launch: N`), or None."""
+def profiled_traffic(path, form):
+    """HBM bytes per launch of a split-form GEMM kernel from the committed PMC summary
+    (a line `traffic_bytes_per_launch[<form>]: N`), or None."""
+    key = f"traffic_bytes_per_launch[{form}]:"
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key):
                     return int(float(line.split(":", 1)[1]))
     except OSError:
         pass
@@ -384,8 +385,15 @@ def batched_case(dt, P=10000, seed=4, loop=True):
     ms_graph = e0.elapsed_ms(e1) / 10
     flops = float((2 * dims[:, 0] * dims[:, 1] * dims[:, 2]).sum())
     nbytes = float(es * (sa + sb + sc).sum())
+    # per-problem roofline: each problem's time at peak is the larger of its MFMA-peak
+    # time and its HBM-peak time (A and B read once, C written once); summed over the
+    # batch
+    peak = PEAK_FP32 if dt == np.float32 else PEAK_FP64
+    pf = 2.0 * dims[:, 0] * dims[:, 1] * dims[:, 2]
+    roof_ms = float(np.maximum(pf / (peak * 1e9), es * (sa + sb + sc) / (PEAK_HBM * 1e6)).sum())
     out = {"problems": P, "graph_ms": round(ms_graph, 4), "tflops": round(flops / ms_graph / 1e9, 2),
-           "flops": flops, "bytes": nbytes}
+           "flops": flops, "bytes": nbytes, "frac_of_mfma_peak": round(flops / ms_graph / 1e9 / peak, 4),
+           "roofline_ms": round(roof_ms, 4), "frac_of_roofline": round(roof_ms / ms_graph, 4)}
     if loop:
         outs = C.to_numpy()
         hp = [(int(m), int(n), int(k), A.ptr + es * int(a), B.ptr + es * int(b), C.ptr + es * int(c))
@@ -910,7 +918,7 @@ def main():
     value = flops / hr["ms"] / 1e9
     achieved = flops / hr["kernel_ms"] / 1e9
     peak = PEAK_BF16 / 3.0  # FP16 and BF16 share the dense MFMA peak
-    traffic = profiled_traffic(TRAFFIC_PROFILES[head])
+    traffic = profiled_traffic(TRAFFIC_PROFILE, head)
     kernel_desc = {"bf16x3": "gemm_dma_kernel<TBF3, 256x256x32, 8 waves> (three BF16 16x16x32 MFMAs per k step)",
                    "fp16x3": "gemm_dma_kernel<TF16X3, 256x256x32, 8 waves> (three FP16 16x16x32 MFMAs per k step)"}
     line = {
@@ -938,7 +946,7 @@ def main():
         "error_vs_fp64": {"plain_fp32": plain["error"], **{f: forms[f]["error_vs_fp64"] for f in forms}},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": round(peak, 2), "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": traffic,
-                     "traffic_source": os.path.relpath(TRAFFIC_PROFILES[head], ROOT) if traffic else None,
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
                      "kernel": kernel_desc[head] + "; work = 2n^3 FP32-equivalent flops per launch; peak = 2500 "
                                "TFLOP/s dense 16-bit MFMA / 3 products",
                      "kernel_ms": round(hr["kernel_ms"], 4)},
