@@ -430,7 +430,64 @@ static int trsv_impl(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const
 // with n < 0 or lda < max(1, n) is skipped and flagged in the stream's error word.
 constexpr int kBatchMaxN = 4096;  // 32 KiB of doubles in LDS
 
-__global__ __launch_bounds__(256) void trsv_batched_kernel(int uplo, int trans, int unit, int64_t batch,
+// One problem's solve by one workgroup.  IN_LDS: the working x is xsol (LDS), else
+// x itself in global memory; RC: the layout of M (1: trans == 0), so that the tile
+// loads are the unconditional clamped ones.  Templates, so that no branch on either
+// sits in the inner loops: a branch join after a global load makes the compiler wait
+// for every load in flight.
+template <bool IN_LDS, int RC>
+__device__ __forceinline__ void batched_solve(const TriView<double> &v, double *x, int64_t n, int unit,
+                                              double (&T)[NB][NB + 1], double *xsol, double (&part)[4][NB]) {
+  const int t = threadIdx.x, lane = t & 63, q = t >> 6;
+  auto xget = [&](int64_t i) -> double { return IN_LDS ? xsol[i] : x[v.map(i)]; };
+  if (IN_LDS) {
+    for (int64_t i = t; i < n; i += 256) xsol[i] = x[v.map(i)];
+    __syncthreads();
+  }
+  const int nbp = (int)cdiv(n, NB);
+  double reg[16];
+  for (int b = 0; b < nbp; ++b) {
+    const int64_t i0 = (int64_t)b * NB;
+    double acc = 0.0;
+    for (int j = 0; j < b; ++j) {
+      load_tile<RC>(v, i0, (int64_t)j * NB, false, reg);
+      store_tile<RC>(v, T, reg, i0, (int64_t)j * NB, false);
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int c = q * 16 + k;
+        acc = fma(T[c][lane], xget((int64_t)j * NB + c), acc);  // j * NB + c < i0 <= n: solved
+      }
+      __syncthreads();
+    }
+    part[q][lane] = acc;
+    load_tile<RC>(v, i0, i0, true, reg);
+    store_tile<RC>(v, T, reg, i0, i0, true);
+    __syncthreads();
+    if (q == 0) {
+      const int64_t i = i0 + lane;
+      const int nrows = (int)std::min<int64_t>(NB, n - i0);
+      double rhs = 0.0, dval = 0.0;
+      if (i < n) {
+        rhs = xget(i) - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
+        if (!unit) dval = v.get(i, i);
+      }
+      double xr = solve_diag_block(rhs, [&](int c) { return T[c][lane]; }, dval, unit, nrows);
+      if (i < n) {
+        if (IN_LDS) xsol[i] = xr;
+        else x[v.map(i)] = xr;
+      }
+    }
+    __syncthreads();
+  }
+  if (IN_LDS)
+    for (int64_t i = t; i < n; i += 256) x[v.map(i)] = xsol[i];
+  __syncthreads();
+}
+
+// Two workgroups per CU (the LDS allows two): the bound keeps the registers within
+// two waves per SIMD.
+__global__ __launch_bounds__(256, 2) void trsv_batched_kernel(int uplo, int trans, int unit, int64_t batch,
                                                            const int32_t *ns, const double *const *As,
                                                            const int32_t *ldas, double *const *xs_g, int *ctrl,
                                                            int *err) {
@@ -438,7 +495,7 @@ __global__ __launch_bounds__(256) void trsv_batched_kernel(int uplo, int trans, 
   __shared__ double xsol[kBatchMaxN];
   __shared__ double part[4][NB];
   __shared__ int psh;
-  const int t = threadIdx.x, lane = t & 63, q = t >> 6;
+  const int t = threadIdx.x;
   for (;;) {
     if (t == 0) psh = atomicAdd(&ctrl[0], 1);
     __syncthreads();
@@ -459,58 +516,14 @@ __global__ __launch_bounds__(256) void trsv_batched_kernel(int uplo, int trans, 
     v.trans = trans == CORAL_CONJUGATE_TRANSPOSE ? 1 : trans;
     const bool lower = uplo == CORAL_LOWER_TRIANGULAR;
     v.fwd = (lower == (trans == CORAL_NO_TRANSPOSE)) ? 1 : 0;
-    double *x = xs_g[p];
-    const bool in_lds = n <= kBatchMaxN;  // uniform over the workgroup
-    if (in_lds) {
-      for (int64_t i = t; i < n; i += 256) xsol[i] = x[v.map(i)];
-      __syncthreads();
+    // uniform branches: n and the layout are the same for the whole workgroup
+    if (n <= kBatchMaxN) {
+      if (v.trans == 0) batched_solve<true, 1>(v, xs_g[p], n, unit, T, xsol, part);
+      else batched_solve<true, 0>(v, xs_g[p], n, unit, T, xsol, part);
+    } else {
+      if (v.trans == 0) batched_solve<false, 1>(v, xs_g[p], n, unit, T, xsol, part);
+      else batched_solve<false, 0>(v, xs_g[p], n, unit, T, xsol, part);
     }
-    const int nbp = (int)cdiv(n, NB);
-    double reg[16];
-    for (int b = 0; b < nbp; ++b) {
-      const int64_t i0 = (int64_t)b * NB;
-      double acc = 0.0;
-      for (int j = 0; j < b; ++j) {
-        load_tile(v, i0, (int64_t)j * NB, false, reg);
-        store_tile(v, T, reg, i0, (int64_t)j * NB, false);
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-          const int c = q * 16 + k;
-          const int64_t jc = (int64_t)j * NB + c;  // < i0 <= n: a solved entry
-          double xv;
-          if (in_lds) xv = xsol[jc];
-          else xv = x[v.map(jc)];
-          acc = fma(T[c][lane], xv, acc);
-        }
-        __syncthreads();
-      }
-      part[q][lane] = acc;
-      load_tile(v, i0, i0, true, reg);
-      store_tile(v, T, reg, i0, i0, true);
-      __syncthreads();
-      if (q == 0) {
-        const int64_t i = i0 + lane;
-        const int nrows = (int)std::min<int64_t>(NB, n - i0);
-        double rhs = 0.0, dval = 0.0;
-        if (i < n) {
-          double xi;
-          if (in_lds) xi = xsol[i];
-          else xi = x[v.map(i)];
-          rhs = xi - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
-          if (!unit) dval = v.get(i, i);
-        }
-        double xr = solve_diag_block(rhs, [&](int c) { return T[c][lane]; }, dval, unit, nrows);
-        if (i < n) {
-          if (in_lds) xsol[i] = xr;
-          else x[v.map(i)] = xr;
-        }
-      }
-      __syncthreads();
-    }
-    if (in_lds)
-      for (int64_t i = t; i < n; i += 256) x[v.map(i)] = xsol[i];
-    __syncthreads();
   }
 }
 
