@@ -592,6 +592,7 @@ def suite_rows(parts=("gemv", "symv", "tr", "rank", "l1")):
         del A, x, y
     # ---- SYMV / HEMV (a10, a11): stored triangle ----
     for dt, name, n, uplo in () if "symv" not in parts else ((np.float32, "ssymv", 16384, 0), (np.float64, "dsymv", 16384, 0),
+                              (np.float64, "dsymv", 16384, 1), (np.complex64, "chemv", 16384, 0),
                               (np.complex128, "zhemv", 8192, 1)):
         es = np.dtype(dt).itemsize
         A, x, y = device.DeviceArray(n * n, dt), device.DeviceArray(n, dt), device.DeviceArray(n, dt)

@@ -100,6 +100,46 @@ def test_symv_hemv(gpu_lib, p, uplo):
     assert np.abs(o_u - o_l).max() <= (1e-5 if p in "sc" else 1e-13) * max(1, np.abs(o_u).max())
 
 
+@pytest.mark.parametrize("cfg", ["blocks", None, "4,1", "2,2", "2,5"],
+                         ids=["blocks", "stream", "stream-nv4", "stream-wg2", "stream-wg5"])
+@pytest.mark.parametrize("p", "sdcz")
+def test_symv_hemv_row_groups(gpu_lib, p, cfg, monkeypatch):
+    """Orders that span many row groups of the streaming kernel (a row group is 64 x NV x
+    (16 / element size) rows: 512 for float with two vectors per column, 128 for complex
+    double), ragged against row groups, column runs and four-column groups, with padded
+    lda, strided x and y, NaN in the unstored triangle, both triangles: by 64 x 64 blocks
+    (these orders' default) and streamed (CORAL_SYMV_STREAM_MIN_N=0) in every shape
+    CORAL_SYMV_CFG selects (vectors per column, workgroups per CU, which sets the run
+    length); two calls give the same bits."""
+    if cfg != "blocks":
+        monkeypatch.setenv("CORAL_SYMV_STREAM_MIN_N", "0")
+        if cfg:
+            monkeypatch.setenv("CORAL_SYMV_CFG", cfg)
+    herm = p in "cz"
+    f = DT[p]
+    fn = {"s": blas.ssymv, "d": blas.dsymv, "c": blas.chemv, "z": blas.zhemv}[p]
+    rng = np.random.default_rng(22)
+    for uplo in (0, 1):
+        for n, lda, incx, incy in ((2049, 2051, 2, 1), (4133, 4133, 1, 3)):
+            A = rand(rng, lda * n, f).reshape(n, lda).T.copy()
+            unstored = np.tril(np.ones((n, n), bool), -1) if uplo == 0 else np.triu(np.ones((n, n), bool), 1)
+            A[:n][unstored] = np.nan
+            a = A.T.reshape(-1).copy()
+            x = rand(rng, (n - 1) * incx + 1, f)
+            y = rand(rng, (n - 1) * incy + 1, f)
+            al, be = ((0.5, 0.75), (-0.25, 0.5)) if herm else (0.75, -0.5)
+            out = y.copy()
+            fn(uplo, n, al, a, lda, x, incx, be, out, incy)
+            again = y.copy()
+            fn(uplo, n, al, a, lda, x, incx, be, again, incy)
+            assert np.array_equal(out, again)
+            ref = ref64.symv(uplo, n, al, a, lda, x, incx, be, y, incy, herm=herm)
+            assert np.all(np.isfinite(out))
+            rtol = 2e-5 if p in "sc" else 1e-12
+            err = np.abs(out - ref).max() / (np.abs(ref).max() + 1)
+            assert err <= rtol * np.sqrt(n) / 8, (uplo, n, err)
+
+
 def tri(n, lda, f, rng, uplo, cplx):
     a = np.full(lda * n, np.nan, f)
     for j in range(n):
@@ -150,16 +190,24 @@ def _tri_fast(n, lda, f, rng, uplo, cplx):
     return a.reshape(-1)
 
 
+@pytest.mark.parametrize("path", ["blocks", "stream"])
 @pytest.mark.parametrize("p", "sdcz")
-def test_trmv_against_fp64(gpu_lib, p):
-    """x := op(A) x for every uplo x trans x diag, ragged n around the 64-row blocks,
-    padded lda and strided x: elementwise within 2*n*eps*(|op(A)| |x|) of FP64, with
-    NaN in the unstored triangle, the padding and (unit) the diagonal."""
+def test_trmv_against_fp64(gpu_lib, p, path, monkeypatch):
+    """x := op(A) x for every uplo x trans x diag, ragged n around the streaming
+    kernel's row groups (64 x 2 x 16 / element size rows: 512 float, 256 double or
+    complex float, 128 complex double) and its four-column groups, padded lda (16-byte
+    aligned columns take the unguarded loads, the others the element loads) and strided
+    x: elementwise within 2*n*eps*(|op(A)| |x|) of FP64, with NaN in the unstored
+    triangle, the padding and (unit) the diagonal; by 64 x 64 blocks (these orders'
+    default) and streamed (CORAL_SYMV_STREAM_MIN_N=0)."""
+    if path == "stream":
+        monkeypatch.setenv("CORAL_SYMV_STREAM_MIN_N", "0")
     cplx = p in "cz"
     f = DT[p]
     rng = np.random.default_rng(24)
     eps = 2.0 ** -24 if p in "sc" else 2.0 ** -53
-    for n, lda, incx in ((1, 1, 1), (63, 63, 1), (64, 66, 2), (129, 129, 1), (1000, 1003, 3), (1537, 1537, 1)):
+    for n, lda, incx in ((1, 1, 1), (63, 63, 1), (64, 66, 2), (129, 129, 1), (1000, 1003, 3), (1537, 1537, 1),
+                         (2048, 2048, 1), (2501, 2504, 2)):
         for uplo in (0, 1):
             a = _tri_fast(n, lda, f, rng, uplo, cplx)
             for trans in (0, 1, 2):
