@@ -4,10 +4,11 @@
 // No reference counterpart: in CORAL the equivalent is a loop of sgemm/dgemm calls
 // (coral-aarch64/src/level3/sgemm.rs:106, dgemm.rs:107), each with its own packing
 // buffers.  Here:
-//   1. batched_setup_kernel (one workgroup of 1024 threads) reads the per-problem
-//      dims from device memory, checks them, picks a tile kind per problem from the
-//      (m, n, k) bucket table the autotuner fills, writes the exclusive prefix sum of
-//      tile counts (int64) and resets the queue head -- with kernel stores only, so
+//   1. batched_count_kernel (a thread per problem) reads the per-problem dims from
+//      device memory, checks them, picks a tile kind per problem from the (m, n, k)
+//      bucket table the autotuner fills and writes its tile count; batched_scan_kernel
+//      (one workgroup of 1024 threads) turns the counts into per-queue exclusive
+//      prefix sums (int64) and resets the queue heads -- with kernel stores only, so
 //      the sequence can be captured in a hipGraph and replayed;
 //   2. batched_wg_kernel: persistent workgroups of 2 x 2 wavefronts.  A workgroup
 //      takes one work item (a C tile) at a time with an atomicAdd on its queue head,
@@ -39,10 +40,11 @@ constexpr int kNB = 16;  // log2 buckets per dimension (dims up to 2^15; larger 
 constexpr int kTableSize = 2 * 4 * kNB * kNB * kNB;  // [dtype][op pair][bm][bn][bk]
 __device__ int8_t g_kind_table[kTableSize];
 
+// The smallest b with 2^b >= x, at most kNB - 1 (0 for x <= 1).
 __host__ __device__ inline int bucket_of(int x) {
-  int b = 0;
-  while (b < kNB - 1 && (1 << b) < x) ++b;
-  return b;
+  if (x <= 1) return 0;
+  const int b = 32 - __builtin_clz((unsigned)(x - 1));
+  return b < kNB - 1 ? b : kNB - 1;
 }
 constexpr int kKinds = 4;
 // Problem p belongs to queue p % kXcd; workgroup w takes items from queue w % kXcd
@@ -54,6 +56,10 @@ constexpr int kXcd = 8;
 constexpr int kHeadStride = 16;  // 128 bytes between queue heads
 __host__ __device__ inline int kind_rows(int kind) { return kind == 0 ? 32 : (kind == 2 ? 128 : 64); }
 __host__ __device__ inline int kind_cols(int kind) { return kind == 0 ? 32 : (kind == 3 ? 128 : 64); }
+// log2 of kind_rows / kind_cols (tile counts by shifts: a 64-bit division is a long
+// software routine on the GPU)
+__host__ __device__ inline int kind_rows_log2(int kind) { return kind == 0 ? 5 : (kind == 2 ? 7 : 6); }
+__host__ __device__ inline int kind_cols_log2(int kind) { return kind == 0 ? 5 : (kind == 3 ? 7 : 6); }
 // Without a tuned entry: 32x32 tiles for problems within 32x32, 128x64 / 64x128 once
 // the long side reaches 128 and the short one 64 (fewer operand bytes per MFMA),
 // 64x64 otherwise.
@@ -85,72 +91,116 @@ struct BatchArgs {
 // kMaxLd: the tile kernel addresses A and B with 32-bit per-lane byte offsets of up to
 // 15 ld + 15 elements.
 constexpr int64_t kMaxLd = int64_t(1) << 24;
-__device__ __forceinline__ bool problem_valid(const BatchArgs &g, int64_t p) {
-  const int64_t m = g.m[p], n = g.n[p], k = g.k[p];
+__device__ __forceinline__ bool problem_valid(const BatchArgs &g, int64_t m, int64_t n, int64_t k, int64_t lda,
+                                              int64_t ldb, int64_t ldc) {
   if (m < 0 || n < 0 || k < 0) return false;
   const int64_t arows = g.op_a == CORAL_NO_TRANSPOSE ? m : k;
   const int64_t brows = g.op_b == CORAL_NO_TRANSPOSE ? k : n;
-  const int64_t lda = g.lda[p], ldb = g.ldb[p];
-  return lda >= imax64(1, arows) && ldb >= imax64(1, brows) && g.ldc[p] >= imax64(1, m) && lda <= kMaxLd &&
-         ldb <= kMaxLd;
+  return lda >= imax64(1, arows) && ldb >= imax64(1, brows) && ldc >= imax64(1, m) && lda <= kMaxLd && ldb <= kMaxLd;
 }
 
-// Rounds of 1024 consecutive problems (coalesced reads of the dims): each thread
-// takes one problem, and a segmented inclusive scan (the problems of one queue are
-// the lanes l with equal l % kXcd; then across the 16 waves) turns the tile counts
-// into per-queue exclusive prefix sums carried across rounds.
-__global__ __launch_bounds__(1024) void batched_setup_kernel(BatchArgs g) {
-  static_assert(64 % kXcd == 0 && 1024 % kXcd == 0, "queues are lane classes");
-  __shared__ int64_t wsum[16][kXcd];
-  __shared__ int64_t s_run[kXcd];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6, q = lane % kXcd;
-  if (t < kXcd) s_run[t] = 0;
+// Setup, in two launches:
+//  1. batched_count_kernel, one thread per problem: the problem is checked, gets its
+//     tile kind, and its tile count is written to offsets[p].
+//  2. batched_scan_kernel, one workgroup of 1024 threads: queue q's problems q,
+//     q + kXcd, ... are split into 128 contiguous shares, one per thread of the queue's
+//     128 threads (two wavefronts).  Each thread sums its share's counts (kShare loads
+//     in flight at a time), the 128 share totals are scanned once (wave shuffles, then
+//     the two wavefronts' totals through LDS), and each thread writes the exclusive
+//     prefix sums of its share over the counts (from registers when the share fits in
+//     one chunk of kShare, else re-reading them).  It also resets the queue heads.
+constexpr int kSetupThreads = 1024;
+constexpr int kQueueThreads = kSetupThreads / kXcd;  // 128: two wavefronts per queue
+constexpr int kShare = 16;                            // counts one thread holds at once
+
+struct ProblemDims {
+  int32_t m, n, k, lda, ldb, ldc;
+};
+// Loads from an index clamped into the batch, so no load sits behind a branch.
+__device__ __forceinline__ ProblemDims load_dims(const BatchArgs &g, int64_t p) {
+  const int64_t pc = p < g.batch ? p : g.batch - 1;
+  return ProblemDims{g.m[pc], g.n[pc], g.k[pc], g.lda[pc], g.ldb[pc], g.ldc[pc]};
+}
+
+__device__ __forceinline__ int64_t problem_tiles(const BatchArgs &g, int64_t p, const ProblemDims &d, bool &bad) {
+  const bool in = p < g.batch;
+  const bool valid = problem_valid(g, d.m, d.n, d.k, d.lda, d.ldb, d.ldc);
+  bad |= in && !valid;
+  const bool use = in && valid && d.m > 0 && d.n > 0;
+  int idx = g.table_base + (bucket_of(d.m) * kNB + bucket_of(d.n)) * kNB + bucket_of(d.k > 0 ? d.k : 1);
+  idx = use ? idx : g.table_base;  // an index inside the table either way
+  int kind = g_kind_table[idx];
+  if (kind < 0 || kind >= kKinds) kind = default_kind(d.m, d.n, d.k);
+  kind = use ? kind : 1;
+  if (in) g.kinds[p] = (int8_t)kind;
+  const int64_t rt = ((int64_t)d.m + kind_rows(kind) - 1) >> kind_rows_log2(kind);
+  const int64_t ct = ((int64_t)d.n + kind_cols(kind) - 1) >> kind_cols_log2(kind);
+  return use ? rt * ct : 0;
+}
+
+__global__ __launch_bounds__(256) void batched_count_kernel(BatchArgs g) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   bool bad = false;
-  __syncthreads();
-  for (int64_t base = 0; base < g.batch; base += 1024) {
-    const int64_t p = base + t;  // p % kXcd == q
-    int64_t tiles = 0;
-    if (p < g.batch) {
-      int kind = 1;
-      if (!problem_valid(g, p)) {
-        bad = true;
-      } else {
-        const int m = g.m[p], n = g.n[p], k = g.k[p];
-        if (m > 0 && n > 0) {
-          const int idx = g.table_base + (bucket_of(m) * kNB + bucket_of(n)) * kNB + bucket_of(k > 0 ? k : 1);
-          kind = g_kind_table[idx];
-          if (kind < 0 || kind >= kKinds) kind = default_kind(m, n, k);
-          const int64_t tr = kind_rows(kind), tc = kind_cols(kind);
-          tiles = ((m + tr - 1) / tr) * ((n + tc - 1) / tc);
-        }
-      }
-      g.kinds[p] = (int8_t)kind;
-    }
-    int64_t v = tiles;  // inclusive scan over this wave's lanes of queue q
-#pragma unroll
-    for (int off = kXcd; off < 64; off <<= 1) {
-      const int64_t u = __shfl_up(v, off, 64);
-      if (lane >= off) v += u;
-    }
-    if (lane >= 64 - kXcd) wsum[w][q] = v;  // the wave's total for queue q
-    __syncthreads();
-    if (t < kXcd) {  // inclusive scan over the waves, one queue per thread
-      int64_t run = 0;
-      for (int i = 0; i < 16; ++i) {
-        run += wsum[i][t];
-        wsum[i][t] = run;
-      }
-    }
-    __syncthreads();
-    if (p < g.batch) g.offsets[p] = s_run[q] + (w > 0 ? wsum[w - 1][q] : 0) + v - tiles;
-    __syncthreads();  // every thread has read s_run and wsum
-    if (t < kXcd) s_run[t] += wsum[15][t];
-    __syncthreads();
-  }
+  const ProblemDims d = load_dims(g, p);
+  const int64_t tiles = problem_tiles(g, p, d, bad);
+  if (p < g.batch) g.offsets[p] = tiles;
   if (bad) atomicOr(g.err, (int)ERRF_PROBLEM_ARG);
-  if (t < kXcd) {
-    g.offsets[g.batch + t] = s_run[t];
-    g.head[t * kHeadStride] = 0ull;
+}
+
+__global__ __launch_bounds__(kSetupThreads) void batched_scan_kernel(BatchArgs g) {
+  static_assert(kQueueThreads == 128, "two wavefronts per queue");
+  __shared__ int64_t wtot[kXcd][2];  // [queue][wavefront of the queue]: inclusive share totals
+  const int t = threadIdx.x, lane = t & 63;
+  const int q = t / kQueueThreads, c = t % kQueueThreads, hw = c / 64;  // queue, share, its wavefront
+  const int64_t nq = g.batch > q ? (g.batch - q + kXcd - 1) / kXcd : 0;  // problems in queue q
+  const int64_t len = (nq + kQueueThreads - 1) / kQueueThreads;          // share length (uniform per queue)
+  const int64_t j0 = (int64_t)c * len, j1 = j0 + len < nq ? j0 + len : nq;
+  auto at = [&](int64_t j) -> int64_t & { return g.offsets[q + (int64_t)kXcd * j]; };
+  int64_t v[kShare];
+  int64_t sum = 0;
+  for (int64_t cb = 0; cb < len; cb += kShare) {  // pass A: the share's total
+#pragma unroll
+    for (int i = 0; i < kShare; ++i) {
+      const int64_t j = j0 + cb + i;
+      v[i] = (cb + i < len && j < j1) ? at(j) : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < kShare; ++i) sum += v[i];
+  }
+  int64_t incl = sum;  // inclusive scan of the share totals over the queue's 128 threads
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int64_t u = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += u;
+  }
+  if (lane == 63) wtot[q][hw] = incl;
+  __syncthreads();
+  int64_t run = (hw ? wtot[q][0] : 0) + incl - sum;  // exclusive prefix of the share
+  if (len <= kShare) {  // pass B from registers (v holds the whole share)
+#pragma unroll
+    for (int i = 0; i < kShare; ++i) {
+      const int64_t j = j0 + i;
+      if (i < len && j < j1) at(j) = run;
+      run += v[i];
+    }
+  } else {  // pass B re-reading the counts, chunk by chunk
+    for (int64_t cb = 0; cb < len; cb += kShare) {
+#pragma unroll
+      for (int i = 0; i < kShare; ++i) {
+        const int64_t j = j0 + cb + i;
+        v[i] = (cb + i < len && j < j1) ? at(j) : 0;
+      }
+#pragma unroll
+      for (int i = 0; i < kShare; ++i) {
+        const int64_t j = j0 + cb + i;
+        if (cb + i < len && j < j1) at(j) = run;
+        run += v[i];
+      }
+    }
+  }
+  if (c == 0) {
+    g.offsets[g.batch + q] = wtot[q][0] + wtot[q][1];
+    g.head[q * kHeadStride] = 0ull;
   }
 }
 
@@ -683,7 +733,9 @@ static int batched_impl(int dt, int32_t op_a, int32_t op_b, int64_t batch, const
   g.head = (unsigned long long *)((char *)ws + head_at);
   g.err = err;
   g.table_base = ((dt * 4) + (g.op_a != 0) * 2 + (g.op_b != 0)) * kNB * kNB * kNB;
-  hipLaunchKernelGGL(batched_setup_kernel, dim3(1), dim3(1024), 0, s, g);
+  hipLaunchKernelGGL(batched_count_kernel, dim3((unsigned)cdiv(batch, 256)), dim3(256), 0, s, g);
+  CORAL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(batched_scan_kernel, dim3(1), dim3(kSetupThreads), 0, s, g);
   CORAL_LAUNCH_CHECK();
   // op(A) rows k-contiguous (Transpose) / op(B) columns k-contiguous (NoTranspose)
   const int layout = (g.op_a != CORAL_NO_TRANSPOSE ? 2 : 0) + (g.op_b == CORAL_NO_TRANSPOSE ? 1 : 0);

@@ -111,6 +111,18 @@ def test_batched_uneven(gpu_lib, dtype, eps, ops):
     assert check(batch, 1.25, 0.5, eps) <= 1.0
 
 
+def test_batched_queue_shares_longer_than_one_chunk(gpu_lib):
+    """20,000 small problems: each of the 8 work queues holds 2,500, more than the setup
+    kernel's 128 threads x 16 counts per queue in registers, so its per-queue prefix sums
+    re-read the counts chunk by chunk; every problem is checked against FP64."""
+    rng = np.random.default_rng(55)
+    dims = [tuple(int(v) for v in rng.integers(1, 24, 3)) for _ in range(20000)]
+    batch = Batch(dims, np.float32, 0, 0, rng)
+    batch.run(1.0, 0.0)
+    device.synchronize()
+    assert check(batch, 1.0, 0.0, 2.0 ** -24) <= 1.0
+
+
 def test_batched_alpha_zero_and_beta_zero(gpu_lib):
     rng = np.random.default_rng(50)
     dims = log_uniform_dims(rng, 50, 1, 100)
