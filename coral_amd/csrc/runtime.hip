@@ -342,8 +342,8 @@ int coral_stream_check(coral_stream_t stream) {
   return (flags & ERRF_WAIT_BOUND) ? CORAL_ERR_WAIT_BOUND : CORAL_ERR_PROBLEM_ARG;
 }
 int coral_testing_set_wait_limit(int64_t spins) {
-  if (spins < 0 || spins > (int64_t)UINT32_MAX) return -1;
-  g_wait_limit = spins == 0 ? (1u << 22) : (uint32_t)spins;
+  if (spins < -1 || spins > (int64_t)UINT32_MAX) return -1;
+  g_wait_limit = spins == 0 ? (1u << 22) : (spins < 0 ? 0u : (uint32_t)spins);
   return CORAL_OK;
 }
 int coral_stream_synchronize(coral_stream_t stream) {

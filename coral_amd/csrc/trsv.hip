@@ -228,15 +228,107 @@ __global__ void stage_init_kernel(W *st, int64_t nwords, W sentinel, int *ctrl) 
   for (int64_t i = i0; i < nwords; i += (int64_t)gridDim.x * blockDim.x) st[i] = sentinel;
 }
 
+// Sum of a value over the four lanes of each quad (DPP quad_perm, word by word).
+template <typename E> __device__ __forceinline__ E quad_sum(E v) {
+  constexpr int NW = sizeof(E) / 4;
+  uint32_t w[NW], u[NW];
+  __builtin_memcpy(w, &v, sizeof(E));
+#pragma unroll
+  for (int i = 0; i < NW; ++i) u[i] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)w[i], 0xB1, 0xF, 0xF, false);
+  E o;
+  __builtin_memcpy(&o, u, sizeof(E));
+  v = eadd(v, o);  // lanes l, l ^ 1
+  __builtin_memcpy(w, &v, sizeof(E));
+#pragma unroll
+  for (int i = 0; i < NW; ++i) u[i] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)w[i], 0x4E, 0xF, 0xF, false);
+  __builtin_memcpy(&o, u, sizeof(E));
+  return eadd(v, o);  // and l ^ 2
+}
+
+// inv(L) of the 64 x 64 lower-triangular diagonal block, in place.  On entry Td[c][r]
+// holds L(r, c) for c < r (zero elsewhere) and dg[r] = L(r, r) (1 for a unit
+// diagonal or a row past n); on exit Td[c][r] = inv(L)(r, c) for c <= r.  S is 48 x
+// 16 elements of scratch (in T).  In 16 x 16 blocks:
+//  1. wavefront w inverts diagonal block w by forward substitution: lane = 4 cj + p
+//     holds column cj of it at rows p, p + 4, p + 8, p + 12; row i's dot product is
+//     split over the four phases p and summed across the quad, then divided by its
+//     diagonal (rows above the column stay zero, and no term of them is formed, as
+//     in substitution);
+//  2. block rows 1, 2, 3 in turn: X_ij = -X_ii (sum over k = j .. i-1 of L_ik X_kj)
+//     for every j < i, the sums first (into S), then the products over X_ii, with a
+//     barrier after each (X_ij overwrites L_ij, which the sums of row i read).
+// 16 sequential steps and three levels of 16 x 16 products replace a 64-step
+// substitution whose every step broadcast a row through the scalar unit.
+template <typename E>
+__device__ __forceinline__ void invert_lower64(E (*Td)[NB + 1], const E *dg, E (*S)[NB + 1]) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int cj = lane >> 2, p = lane & 3, c = 16 * w + cj;
+  E Y[4] = {ezero<E>(), ezero<E>(), ezero<E>(), ezero<E>()};
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    E s = ezero<E>();
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int k = p + 4 * m;
+      if (4 * m < i) {  // k < i is possible for some phase
+        const E l = Td[16 * w + k][16 * w + i];
+        const E term = emul(l, Y[m]);
+        s = (k < i && k >= cj) ? eadd(s, term) : s;
+      }
+    }
+    s = quad_sum(s);
+    const E d = dg[16 * w + i];
+    const E num = esub(i == cj ? eone<E>() : ezero<E>(), s);
+    const E y = i >= cj ? qdiv(num, d, erecip(d)) : ezero<E>();
+    if (p == (i & 3)) Y[i >> 2] = y;
+  }
+  // every lane has read its block of L: write the block's inverse over it
+#pragma unroll
+  for (int m = 0; m < 4; ++m) Td[c][16 * w + p + 4 * m] = Y[m];
+  __syncthreads();
+  const int r = t & 15, cc = t >> 4;  // one entry (r, cc) of a 16 x 16 block per thread
+#pragma unroll
+  for (int i = 1; i < 4; ++i) {
+    for (int j = 0; j < i; ++j) {  // S_j = sum_k L_ik X_kj
+      E acc = ezero<E>();
+      for (int k = j; k < i; ++k)
+#pragma unroll
+        for (int l = 0; l < 16; ++l) acc = emad(acc, Td[16 * k + l][16 * i + r], Td[16 * j + cc][16 * k + l]);
+      S[16 * j + cc][r] = acc;
+    }
+    __syncthreads();
+    for (int j = 0; j < i; ++j) {  // X_ij = -X_ii S_j
+      E acc = ezero<E>();
+#pragma unroll
+      for (int l = 0; l < 16; ++l) acc = emad(acc, Td[16 * i + l][16 * i + r], S[16 * j + cc][l]);
+      Td[16 * j + cc][16 * i + r] = eneg(acc);
+    }
+    __syncthreads();
+  }
+}
+
 // Persistent forward solve of the virtual lower-triangular M (see the file comment).
 // RC: the layout of M (1: trans == 0, lanes walk rows; 0: lanes walk columns).
+// Block b inverts its diagonal block first (it depends on nothing) while its first
+// tiles load.  Elements of up to 8 bytes (MT) take the last dependency off the
+// workgroup: the block also forms P = inv(L_bb) M(b, b-1) up front (four wavefronts,
+// 16 columns each; wavefront 0 then holds its row of P in registers), accumulates
+// tiles j < b - 1 as block j's values are published, forms y = inv(L_bb) r from the
+// residual r (again over the four wavefronts), and only then waits for block b - 1:
+// x_b = y - P x_{b-1} is one 64-term product in wavefront 0 with no workgroup
+// barrier between the wait and the publish.  Complex double (whose third 64 x 64
+// buffer would not fit in LDS) accumulates every tile j < b over the four
+// wavefronts and multiplies the residual by inv(L_bb) in wavefront 0.
 template <typename E, int RC>
 __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t incx, int unit,
                                                    typename Stage<E>::W *stage, int *ctrl, int nb, int *err,
                                                    uint32_t spin_limit) {
+  constexpr bool MT = sizeof(E) <= 8;
   __shared__ E T[NB][NB + 1];
   __shared__ E Td[NB][NB + 1];
+  __shared__ E Pb[MT ? NB : 1][NB + 1];
   __shared__ E xs[NB];
+  __shared__ E dg[NB];
   __shared__ E part[4][NB];
   __shared__ int bsh;
   const int t = threadIdx.x, lane = t & 63, q = t >> 6;
@@ -245,40 +337,50 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
   const int b = bsh;
   if (b >= nb) return;
   const int64_t i0 = (int64_t)b * NB;
-  // The diagonal block depends on nothing: fetch it first and invert it while the
-  // blocks this one depends on are still being solved.  Wavefront q computes columns
-  // [16q, 16q + 16) of inv(L_bb) by forward substitution on the unit vectors (lane =
-  // row; 64 steps of: finish row k, broadcast it, update the rows below), so the
-  // solve on the critical path is one triangular matrix-vector product.
+  const int nloop = MT ? b - 1 : b;  // tiles accumulated by the four wavefronts
+  // wavefront 0's right-hand side, read now: after the tile loop it would put a
+  // memory round trip on the critical path of every block
+  E rhs0 = ezero<E>();
+  if (q == 0 && i0 + lane < v.n) rhs0 = x[v.map(i0 + lane) * incx];
+  // the first two tiles of the loop (and, MT, the last tile) are in flight while the
+  // diagonal block is inverted
+  E reg0[16], reg1[16], regp[16];
+  if (nloop > 0) load_tile<RC>(v, i0, 0, false, reg0);
+  if (nloop > 1) load_tile<RC>(v, i0, NB, false, reg1);
+  if (MT && b > 0) load_tile<RC>(v, i0, (int64_t)(b - 1) * NB, false, regp);
   {
     E dreg[16];
     load_tile<RC>(v, i0, i0, true, dreg);
     store_tile<RC>(v, Td, dreg, i0, i0, true);
+    if (q == 0) {
+      const int64_t i = i0 + lane;
+      dg[lane] = (unit || i >= v.n) ? eone<E>() : v.get(i, i);
+    }
     __syncthreads();
-    const int64_t i = i0 + lane;
-    const E d = (unit || i >= v.n) ? eone<E>() : v.get(i, i);
-    const E rec = erecip(d);
-    E Y[16];
+    invert_lower64(Td, dg, T);
+  }
+  E prow[MT ? NB : 1];  // wavefront 0, lane r: row r of P
+  if constexpr (MT) {
+    if (b > 0) {
+      store_tile<RC>(v, Pb, regp, i0, (int64_t)(b - 1) * NB, false);  // M(b, b-1)
+      __syncthreads();
+      E m[16];
 #pragma unroll
-    for (int cc = 0; cc < 16; ++cc) Y[cc] = (lane == 16 * q + cc) ? eone<E>() : ezero<E>();
+      for (int k = 0; k < 16; ++k) m[k] = ezero<E>();
+      for (int l = 0; l < NB; ++l) {
+        const E a = Td[l][lane];  // inv(L_bb)(lane, l)
 #pragma unroll
-    for (int k = 0; k < NB; ++k) {
-      if (lane == k) {
-#pragma unroll
-        for (int cc = 0; cc < 16; ++cc) Y[cc] = qdiv(Y[cc], d, rec);
+        for (int k = 0; k < 16; ++k) m[k] = emad(m[k], a, Pb[16 * q + k][l]);
       }
-      const E lik = Td[k][lane];  // L(i, k) for lane i > k
-      E yk[16];
+      __syncthreads();  // every wavefront is done reading M(b, b-1)
 #pragma unroll
-      for (int cc = 0; cc < 16; ++cc) yk[cc] = rlane(Y[cc], k);
-      if (lane > k) {  // rows above k are untouched, as in substitution (no 0 * inf)
+      for (int k = 0; k < 16; ++k) Pb[16 * q + k][lane] = m[k];
+      __syncthreads();
+      if (q == 0) {
 #pragma unroll
-        for (int cc = 0; cc < 16; ++cc) Y[cc] = emad(Y[cc], eneg(lik), yk[cc]);
+        for (int c = 0; c < NB; ++c) prow[c] = Pb[c][lane];
       }
     }
-    __syncthreads();  // every wavefront is done reading Td
-#pragma unroll
-    for (int cc = 0; cc < 16; ++cc) Td[16 * q + cc][lane] = Y[cc];  // Td now holds inv(L_bb)
   }
   E acc = ezero<E>();
   // wavefront 0 keeps the next block's staged values in flight (xnext); a block that
@@ -292,71 +394,101 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
     if (c < v.n) ok = stage_load<E>(stage, c, val);
     return __ballot(!ok) == 0;
   };
+  // Wavefront 0: block j's published values, from xnext or by polling.  Tickets make
+  // this wait finite (block j belongs to a workgroup that started earlier and waits
+  // only on smaller tickets); the spin bound is a backstop that records an error in
+  // the stream's error word (reported by coral_stream_check and by the host-pointer
+  // form) instead of hanging if that invariant were broken.  The default bound is
+  // seconds; a legitimate wait is microseconds.
+  auto wait_block = [&](int j) -> E {
+    E val = xnext;
+    if (spin_limit == 0) {  // the host's test mode: every wait gives up at once
+      if (lane == 0) atomicOr(err, (int)ERRF_WAIT_BOUND);
+    } else if (xnext_j != j) {
+      uint32_t polls = 0;  // failed polls; the bound counts them
+      while (!load_block(j, val)) {
+        if (++polls >= spin_limit) {
+          if (lane == 0) atomicOr(err, (int)ERRF_WAIT_BOUND);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    xnext_j = -1;
+    return val;
+  };
   // Off-diagonal tiles stream through two register slots (slot j&1 holds tile j,
   // loaded two tiles ahead); the unrolled-by-two loop keeps slot indices static.
-  E reg0[16], reg1[16];
   auto step = [&](int j, E(&cur)[16]) {
     __syncthreads();  // every wavefront is done reading T and xs of tile j-1
     store_tile<RC>(v, T, cur, i0, (int64_t)j * NB, false);
     if (q == 0) {
-      E val = xnext;
-      if (xnext_j != j) {
-        // Tickets make this wait finite (block j belongs to a workgroup that started
-        // earlier and waits only on smaller tickets); the spin bound is a backstop
-        // that records an error in the stream's error word (reported by
-        // coral_stream_check and by the host-pointer form) instead of hanging if that
-        // invariant were broken.  The default bound is seconds; a legitimate wait is
-        // microseconds.
-        uint32_t spins = 0;
-        while (!load_block(j, val)) {
-          if (++spins > spin_limit) {
-            if (lane == 0) atomicOr(err, (int)ERRF_WAIT_BOUND);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-        }
-      }
-      xs[lane] = val;
-      xnext_j = -1;
+      xs[lane] = wait_block(j);
       if (j + 1 < b && load_block(j + 1, xnext)) xnext_j = j + 1;
     }
     __syncthreads();
-    if (j + 2 < b) load_tile<RC>(v, i0, (int64_t)(j + 2) * NB, false, cur);
+    if (j + 2 < nloop) load_tile<RC>(v, i0, (int64_t)(j + 2) * NB, false, cur);
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int c = q * 16 + k;
       acc = emad(acc, T[c][lane], xs[c]);
     }
   };
-  if (b > 0) load_tile<RC>(v, i0, 0, false, reg0);
-  if (b > 1) load_tile<RC>(v, i0, NB, false, reg1);
-  for (int j = 0; j < b; j += 2) {
+  for (int j = 0; j < nloop; j += 2) {
     step(j, reg0);
-    if (j + 1 < b) step(j + 1, reg1);
+    if (j + 1 < nloop) step(j + 1, reg1);
   }
   part[q][lane] = acc;
   __syncthreads();
-  if (q == 0) {
-    const int64_t i = i0 + lane;
-    const int nrows = (int)std::min<int64_t>(NB, v.n - i0);
-    (void)nrows;
-    E rhs = ezero<E>();
-    if (i < v.n) {
-      rhs = x[v.map(i) * incx];
-      rhs = esub(rhs, eadd(eadd(part[0][lane], part[1][lane]), eadd(part[2][lane], part[3][lane])));
+  const int64_t i = i0 + lane;
+  if constexpr (MT) {
+    // the residual r = b - sum over j < b-1 (four partial sums in a fixed order), then
+    // y = inv(L_bb) r over the four wavefronts (16 columns each)
+    if (q == 0) {
+      E r = ezero<E>();
+      if (i < v.n) r = esub(rhs0, eadd(eadd(part[0][lane], part[1][lane]), eadd(part[2][lane], part[3][lane])));
+      xs[lane] = r;
     }
-    // x = inv(L_bb) r: lane i forms row i from the broadcast residual (four partial
-    // sums in a fixed order keep the chain short and the result deterministic)
-    xs[lane] = rhs;
-    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wavefront's LDS writes are done
-    __builtin_amdgcn_wave_barrier();
-    E s4[4] = {ezero<E>(), ezero<E>(), ezero<E>(), ezero<E>()};
+    __syncthreads();
+    E yq = ezero<E>();
 #pragma unroll
-    for (int c = 0; c < NB; ++c) s4[c & 3] = emad(s4[c & 3], Td[c][lane], xs[c]);
-    const E xr = eadd(eadd(s4[0], s4[1]), eadd(s4[2], s4[3]));
-    if (i < v.n) {
-      stage_store<E>(stage, i, xr);  // publish first: it is the critical path
-      x[v.map(i) * incx] = xr;
+    for (int k = 0; k < 16; ++k) yq = emad(yq, Td[16 * q + k][lane], xs[16 * q + k]);
+    __syncthreads();  // every wavefront has read xs and the residual's part sums
+    part[q][lane] = yq;
+    __syncthreads();
+    if (q == 0) {
+      E xr = eadd(eadd(part[0][lane], part[1][lane]), eadd(part[2][lane], part[3][lane]));
+      if (b > 0) {
+        xs[lane] = wait_block(b - 1);
+        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wavefront's LDS writes are done
+        __builtin_amdgcn_wave_barrier();
+        E s4[4] = {ezero<E>(), ezero<E>(), ezero<E>(), ezero<E>()};
+#pragma unroll
+        for (int c = 0; c < NB; ++c) s4[c & 3] = emad(s4[c & 3], prow[c], xs[c]);
+        xr = esub(xr, eadd(eadd(s4[0], s4[1]), eadd(s4[2], s4[3])));
+      }
+      if (i < v.n) {
+        stage_store<E>(stage, i, xr);  // publish first: it is the critical path
+        x[v.map(i) * incx] = xr;
+      }
+    }
+  } else {
+    if (q == 0) {
+      E rhs = ezero<E>();
+      if (i < v.n) rhs = esub(rhs0, eadd(eadd(part[0][lane], part[1][lane]), eadd(part[2][lane], part[3][lane])));
+      // x = inv(L_bb) r: lane i forms row i from the broadcast residual (four partial
+      // sums in a fixed order keep the chain short and the result deterministic)
+      xs[lane] = rhs;
+      __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wavefront's LDS writes are done
+      __builtin_amdgcn_wave_barrier();
+      E s4[4] = {ezero<E>(), ezero<E>(), ezero<E>(), ezero<E>()};
+#pragma unroll
+      for (int c = 0; c < NB; ++c) s4[c & 3] = emad(s4[c & 3], Td[c][lane], xs[c]);
+      const E xr = eadd(eadd(s4[0], s4[1]), eadd(s4[2], s4[3]));
+      if (i < v.n) {
+        stage_store<E>(stage, i, xr);  // publish first: it is the critical path
+        x[v.map(i) * incx] = xr;
+      }
     }
   }
 }

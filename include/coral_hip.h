@@ -85,9 +85,10 @@ int coral_stream_synchronize(coral_stream_t stream);
  * they return (their kernels have not run yet); the host-pointer forms check them
  * before returning. */
 int coral_stream_check(coral_stream_t stream);
-/* Test hook: spin bound of the TRSV handoff wait (0 restores the default, about
- * seconds of waiting); a tiny bound makes a wait give up so the error path can be
- * exercised while every kernel still exits normally. */
+/* Test hook: bound on the failed polls of a TRSV handoff wait (0 restores the
+ * default, about seconds of waiting; -1 makes every wait give up at once, found or
+ * not), so the error path can be exercised while every kernel still exits
+ * normally. */
 int coral_testing_set_wait_limit(int64_t spins);
 int coral_event_create(void **event);
 int coral_event_destroy(void *event);

@@ -245,8 +245,9 @@ def test_dtrsv_n8192_backward_error(gpu_lib):
 
 
 def test_trsv_wait_bound_is_reported(gpu_lib):
-    # The host lowers the spin bound of TRSV's handoff wait to one retry, so waits
-    # give up (the kernels still run to the end); the host-pointer form and
+    # The host puts TRSV's handoff wait in its test mode (a bound of -1: every wait
+    # gives up at once and records the error; the kernels still run to the end); the
+    # host-pointer form and
     # coral_stream_check must report CORAL_ERR_WAIT_BOUND, and with the default
     # bound restored the same solve is correct and clean.
     from coral_amd._lib import CoralError, call, status_of
@@ -257,7 +258,7 @@ def test_trsv_wait_bound_is_reported(gpu_lib):
     A = np.tril(rng.uniform(-1, 1, (n, n)) / n, -1) + np.diag(rng.uniform(1, 2, n))
     a = np.asfortranarray(A).reshape(-1, order="F")
     b = rng.uniform(-1, 1, n)
-    call("coral_testing_set_wait_limit", 1)
+    call("coral_testing_set_wait_limit", -1)
     try:
         x = b.copy()
         with pytest.raises(CoralError) as e:
