@@ -15,17 +15,26 @@
 //     workgroup only ever waits on blocks owned by workgroups already running;
 //   * a row block accumulates M(block, j) x_j over earlier blocks j as soon as
 //     block j's values are published (tile loads are issued before the wait);
-//   * the diagonal block is solved by one wavefront (lane = row), in the
-//     element precision (FP64 for D/Z), and published through a sentinel-filled
-//     staging vector whose words are their own ready flags (see Stage below).
+//   * the diagonal block's inverse is formed while the first tiles load (it
+//     depends on nothing), the block's right-hand side minus the off-diagonal sums
+//     is multiplied by it, in the element precision (FP64 for D/Z), and x_b is
+//     published through a sentinel-filled staging vector whose words are their own
+//     ready flags (see Stage below);
+//   * from the seventh block on, the last dependency (ND = 1, the default for
+//     elements up to 8 bytes) or two (ND = 2) are precomputed as products with
+//     that inverse, so the step between a predecessor's publish and this block's
+//     is one 64-long dot product per row in one wavefront.
 #include "coral_common.hpp"
 
 #include <algorithm>
+#include <stdlib.h>
 
 namespace coral {
 namespace tri {
 
 constexpr int NB = 64;
+// Blocks below this index solve their last dependency directly (trsv_kernel).
+constexpr int kDirect = 6;
 
 template <typename E>
 struct TriView {
@@ -212,6 +221,29 @@ __device__ __forceinline__ bool stage_load(const typename Stage<E>::W *st, int64
   __builtin_memcpy(&out, &w[0], sizeof(E));
   return ok;
 }
+// The staged words of element i with no check (the check is stage_final, made only
+// where the value is needed, so the load is not waited for where it is issued).
+template <typename E>
+__device__ __forceinline__ E stage_load_raw(const typename Stage<E>::W *st, int64_t i) {
+  using S = Stage<E>;
+  typename S::W w[S::NW];
+#pragma unroll
+  for (int k = 0; k < S::NW; ++k) w[k] = __hip_atomic_load(&st[i * S::NW + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  E out;
+  __builtin_memcpy(&out, &w[0], sizeof(E));
+  return out;
+}
+template <typename E>
+__device__ __forceinline__ bool stage_final(const E &v) {
+  using S = Stage<E>;
+  typename S::W w[S::NW];
+  __builtin_memcpy(&w[0], &v, sizeof(E));
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < S::NW; ++k) ok = ok && w[k] != S::kSentinel;
+  return ok;
+}
+
 template <typename E>
 __device__ __forceinline__ void stage_store(typename Stage<E>::W *st, int64_t i, E v) {
   using S = Stage<E>;
@@ -307,27 +339,50 @@ __device__ __forceinline__ void invert_lower64(E (*Td)[NB + 1], const E *dg, E (
   }
 }
 
+// sum over c of row(c) * xs[c] for this lane, in four partial sums added in a fixed
+// order.  xs (16-byte aligned, the same for every lane) is read 16 bytes at a time:
+// a wavefront keeps at most 15 LDS reads in flight, so the width of the broadcast
+// reads decides how many round trips the product waits for.
+template <typename E, class ROW>
+__device__ __forceinline__ E row_dot(const ROW &row, const E *xs) {
+  constexpr int EV = 16 / sizeof(E);
+  typedef uint32_t xv4u __attribute__((ext_vector_type(4)));
+  E s4[4] = {ezero<E>(), ezero<E>(), ezero<E>(), ezero<E>()};
+#pragma unroll
+  for (int c0 = 0; c0 < NB; c0 += EV) {
+    const xv4u w = *reinterpret_cast<const xv4u *>(xs + c0);
+    E xv[EV];
+    __builtin_memcpy(xv, &w, 16);
+#pragma unroll
+    for (int e = 0; e < EV; ++e) s4[(c0 + e) & 3] = emad(s4[(c0 + e) & 3], row(c0 + e), xv[e]);
+  }
+  return eadd(eadd(s4[0], s4[1]), eadd(s4[2], s4[3]));
+}
+
 // Persistent forward solve of the virtual lower-triangular M (see the file comment).
 // RC: the layout of M (1: trans == 0, lanes walk rows; 0: lanes walk columns).
 // Block b inverts its diagonal block first (it depends on nothing) while its first
-// tiles load.  Elements of up to 8 bytes (MT) take the last dependency off the
-// workgroup: the block also forms P = inv(L_bb) M(b, b-1) up front (four wavefronts,
-// 16 columns each; wavefront 0 then holds its row of P in registers), accumulates
-// tiles j < b - 1 as block j's values are published, forms y = inv(L_bb) r from the
-// residual r (again over the four wavefronts), and only then waits for block b - 1:
-// x_b = y - P x_{b-1} is one 64-term product in wavefront 0 with no workgroup
-// barrier between the wait and the publish.  Complex double (whose third 64 x 64
-// buffer would not fit in LDS) accumulates every tile j < b over the four
-// wavefronts and multiplies the residual by inv(L_bb) in wavefront 0.
-template <typename E, int RC>
+// tiles load.  With ND > 0, blocks from kDirect on take their last ND dependencies off
+// the workgroup: the block forms P1 = inv(L_bb) M(b, b-1) (and, ND = 2,
+// P2 = inv(L_bb) M(b, b-2)) up front (four wavefronts, 16 columns each; wavefront 0
+// keeps its row of P1 in registers, P2 stays in LDS), accumulates tiles j < b - ND as
+// block j's values are published, forms y = inv(L_bb) r from that residual (again
+// over the four wavefronts), and then wavefront 0 alone waits (ND = 2: for block
+// b - 2, u = y - P2 x_{b-2}) for block b - 1, x_b = u - P1 x_{b-1}, and publishes x_b:
+// after the tile loop no workgroup barrier is crossed.  The direct path (ND = 0, and
+// the first kDirect blocks, whose predecessors publish before the products could be
+// ready) accumulates every tile j < b over the four wavefronts and multiplies the
+// residual by inv(L_bb) in wavefront 0.
+template <typename E, int RC, int ND>
 __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t incx, int unit,
                                                    typename Stage<E>::W *stage, int *ctrl, int nb, int *err,
                                                    uint32_t spin_limit) {
-  constexpr bool MT = sizeof(E) <= 8;
+  static_assert(ND >= 0 && ND <= 2, "the last one or two dependencies, or none");
+  constexpr bool MT = ND > 0;
   __shared__ E T[NB][NB + 1];
   __shared__ E Td[NB][NB + 1];
   __shared__ E Pb[MT ? NB : 1][NB + 1];
-  __shared__ E xs[NB];
+  __shared__ __attribute__((aligned(16))) E xs[NB];
   __shared__ E dg[NB];
   __shared__ E part[4][NB];
   __shared__ int bsh;
@@ -337,17 +392,23 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
   const int b = bsh;
   if (b >= nb) return;
   const int64_t i0 = (int64_t)b * NB;
-  const int nloop = MT ? b - 1 : b;  // tiles accumulated by the four wavefronts
+  // The first kDirect blocks take the direct path: their predecessors publish before
+  // the two products with inv(L_bb) could be ready.
+  const bool pre = MT && b >= kDirect;     // uniform over the workgroup
+  const int nloop = pre ? b - ND : b;      // tiles accumulated by the four wavefronts
   // wavefront 0's right-hand side, read now: after the tile loop it would put a
   // memory round trip on the critical path of every block
   E rhs0 = ezero<E>();
   if (q == 0 && i0 + lane < v.n) rhs0 = x[v.map(i0 + lane) * incx];
-  // the first two tiles of the loop (and, MT, the last tile) are in flight while the
-  // diagonal block is inverted
-  E reg0[16], reg1[16], regp[16];
+  // the first two tiles of the loop (and, MT, the last two tiles) are in flight while
+  // the diagonal block is inverted
+  E reg0[16], reg1[16], regp1[16], regp2[16];
   if (nloop > 0) load_tile<RC>(v, i0, 0, false, reg0);
   if (nloop > 1) load_tile<RC>(v, i0, NB, false, reg1);
-  if (MT && b > 0) load_tile<RC>(v, i0, (int64_t)(b - 1) * NB, false, regp);
+  if (pre) {
+    load_tile<RC>(v, i0, (int64_t)(b - 1) * NB, false, regp1);
+    if (ND == 2) load_tile<RC>(v, i0, (int64_t)(b - 2) * NB, false, regp2);
+  }
   {
     E dreg[16];
     load_tile<RC>(v, i0, i0, true, dreg);
@@ -359,33 +420,47 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
     __syncthreads();
     invert_lower64(Td, dg, T);
   }
-  E prow[MT ? NB : 1];  // wavefront 0, lane r: row r of P
+  // Pb := inv(L_bb) Pb, computed by the four wavefronts (16 columns each) and written
+  // back over its input
+  auto times_inverse = [&]() {
+    E m[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) m[k] = ezero<E>();
+    for (int l = 0; l < NB; ++l) {
+      const E a = Td[l][lane];  // inv(L_bb)(lane, l)
+#pragma unroll
+      for (int k = 0; k < 16; ++k) m[k] = emad(m[k], a, Pb[16 * q + k][l]);
+    }
+    __syncthreads();  // every wavefront is done reading the input
+#pragma unroll
+    for (int k = 0; k < 16; ++k) Pb[16 * q + k][lane] = m[k];
+    __syncthreads();
+  };
+  E prow[MT ? NB : 1];  // wavefront 0, lane r: row r of P1
   if constexpr (MT) {
-    if (b > 0) {
-      store_tile<RC>(v, Pb, regp, i0, (int64_t)(b - 1) * NB, false);  // M(b, b-1)
+    if (pre) {
+      store_tile<RC>(v, Pb, regp1, i0, (int64_t)(b - 1) * NB, false);  // M(b, b-1)
       __syncthreads();
-      E m[16];
-#pragma unroll
-      for (int k = 0; k < 16; ++k) m[k] = ezero<E>();
-      for (int l = 0; l < NB; ++l) {
-        const E a = Td[l][lane];  // inv(L_bb)(lane, l)
-#pragma unroll
-        for (int k = 0; k < 16; ++k) m[k] = emad(m[k], a, Pb[16 * q + k][l]);
-      }
-      __syncthreads();  // every wavefront is done reading M(b, b-1)
-#pragma unroll
-      for (int k = 0; k < 16; ++k) Pb[16 * q + k][lane] = m[k];
-      __syncthreads();
+      times_inverse();
       if (q == 0) {
 #pragma unroll
         for (int c = 0; c < NB; ++c) prow[c] = Pb[c][lane];
       }
+      if constexpr (ND == 2) {
+        __syncthreads();  // wavefront 0 is done reading P1
+        store_tile<RC>(v, Pb, regp2, i0, (int64_t)(b - 2) * NB, false);  // M(b, b-2)
+        __syncthreads();
+        times_inverse();  // Pb = P2
+      }
     }
   }
   E acc = ezero<E>();
-  // wavefront 0 keeps the next block's staged values in flight (xnext); a block that
-  // was already complete when they arrived costs no further load latency
+  // Wavefront 0 keeps the next block's staged values in flight (xnext, loaded right
+  // after the current block's arrive, and checked only when they are needed, so the
+  // load costs no wait of its own); a block that was already complete then costs no
+  // further load latency.
   E xnext = ezero<E>();
+  bool xnext_in = false;  // this lane's row of block xnext_j lies inside the matrix
   int xnext_j = -1;
   auto load_block = [&](int j, E &val) -> bool {  // true when every lane holds a final value
     const int64_t c = (int64_t)j * NB + lane;
@@ -394,17 +469,23 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
     if (c < v.n) ok = stage_load<E>(stage, c, val);
     return __ballot(!ok) == 0;
   };
+  auto prefetch_block = [&](int j) {  // an unconditional load from a clamped index
+    const int64_t c = (int64_t)j * NB + lane;
+    xnext_in = c < v.n;
+    xnext = stage_load_raw<E>(stage, xnext_in ? c : v.n - 1);
+    xnext_j = j;
+  };
   // Wavefront 0: block j's published values, from xnext or by polling.  Tickets make
   // this wait finite (block j belongs to a workgroup that started earlier and waits
-  // only on smaller tickets); the spin bound is a backstop that records an error in
+  // only on smaller tickets); the bounded poll is a backstop that records an error in
   // the stream's error word (reported by coral_stream_check and by the host-pointer
   // form) instead of hanging if that invariant were broken.  The default bound is
   // seconds; a legitimate wait is microseconds.
   auto wait_block = [&](int j) -> E {
-    E val = xnext;
+    E val = xnext_in ? xnext : ezero<E>();
     if (spin_limit == 0) {  // the host's test mode: every wait gives up at once
       if (lane == 0) atomicOr(err, (int)ERRF_WAIT_BOUND);
-    } else if (xnext_j != j) {
+    } else if (xnext_j != j || __ballot(xnext_in && !stage_final(xnext)) != 0) {
       uint32_t polls = 0;  // failed polls; the bound counts them
       while (!load_block(j, val)) {
         if (++polls >= spin_limit) {
@@ -424,7 +505,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
     store_tile<RC>(v, T, cur, i0, (int64_t)j * NB, false);
     if (q == 0) {
       xs[lane] = wait_block(j);
-      if (j + 1 < b && load_block(j + 1, xnext)) xnext_j = j + 1;
+      if (j + 1 < b) prefetch_block(j + 1);
     }
     __syncthreads();
     if (j + 2 < nloop) load_tile<RC>(v, i0, (int64_t)(j + 2) * NB, false, cur);
@@ -441,9 +522,10 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
   part[q][lane] = acc;
   __syncthreads();
   const int64_t i = i0 + lane;
-  if constexpr (MT) {
-    // the residual r = b - sum over j < b-1 (four partial sums in a fixed order), then
-    // y = inv(L_bb) r over the four wavefronts (16 columns each)
+  // four partial sums in a fixed order keep each chain short and the result
+  // deterministic
+  if (pre) {
+    // the residual of tiles j < b - ND, then y = inv(L_bb) r over the four wavefronts
     if (q == 0) {
       E r = ezero<E>();
       if (i < v.n) r = esub(rhs0, eadd(eadd(part[0][lane], part[1][lane]), eadd(part[2][lane], part[3][lane])));
@@ -458,14 +540,19 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
     __syncthreads();
     if (q == 0) {
       E xr = eadd(eadd(part[0][lane], part[1][lane]), eadd(part[2][lane], part[3][lane]));
-      if (b > 0) {
-        xs[lane] = wait_block(b - 1);
+      if constexpr (ND == 2) {  // u = y - P2 x_{b-2}, P2 from LDS
+        xs[lane] = wait_block(b - 2);
+        prefetch_block(b - 1);
         __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wavefront's LDS writes are done
         __builtin_amdgcn_wave_barrier();
-        E s4[4] = {ezero<E>(), ezero<E>(), ezero<E>(), ezero<E>()};
-#pragma unroll
-        for (int c = 0; c < NB; ++c) s4[c & 3] = emad(s4[c & 3], prow[c], xs[c]);
-        xr = esub(xr, eadd(eadd(s4[0], s4[1]), eadd(s4[2], s4[3])));
+        xr = esub(xr, row_dot<E>([&](int c) { return Pb[c][lane]; }, xs));
+        __builtin_amdgcn_wave_barrier();  // every lane has read xs
+      }
+      {  // x_b = u - P1 x_{b-1}, P1 from registers
+        xs[lane] = wait_block(b - 1);
+        __builtin_amdgcn_s_waitcnt(0xc07f);
+        __builtin_amdgcn_wave_barrier();
+        xr = esub(xr, row_dot<E>([&](int c) { return prow[c]; }, xs));
       }
       if (i < v.n) {
         stage_store<E>(stage, i, xr);  // publish first: it is the critical path
@@ -476,15 +563,11 @@ __global__ __launch_bounds__(256) void trsv_kernel(TriView<E> v, E *x, int64_t i
     if (q == 0) {
       E rhs = ezero<E>();
       if (i < v.n) rhs = esub(rhs0, eadd(eadd(part[0][lane], part[1][lane]), eadd(part[2][lane], part[3][lane])));
-      // x = inv(L_bb) r: lane i forms row i from the broadcast residual (four partial
-      // sums in a fixed order keep the chain short and the result deterministic)
+      // x = inv(L_bb) r: lane i forms row i from the broadcast residual
       xs[lane] = rhs;
       __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wavefront's LDS writes are done
       __builtin_amdgcn_wave_barrier();
-      E s4[4] = {ezero<E>(), ezero<E>(), ezero<E>(), ezero<E>()};
-#pragma unroll
-      for (int c = 0; c < NB; ++c) s4[c & 3] = emad(s4[c & 3], Td[c][lane], xs[c]);
-      const E xr = eadd(eadd(s4[0], s4[1]), eadd(s4[2], s4[3]));
+      const E xr = row_dot<E>([&](int c) { return Td[c][lane]; }, xs);
       if (i < v.n) {
         stage_store<E>(stage, i, xr);  // publish first: it is the critical path
         x[v.map(i) * incx] = xr;
@@ -525,6 +608,17 @@ static TriView<E> make_view(int32_t uplo, int32_t trans, const E *a, int64_t lda
   return v;
 }
 
+// How many of a block's last dependencies trsv_kernel precomputes (ND), per element
+// type.  Complex double has no LDS for the products (0).  Measured on one box, lower
+// non-unit n = 8192, N / T, us, for ND = 2 / 1 / 0: STRSV 183 / 138, 178 / 137, 193 / 176;
+// DTRSV 238 / 211, 221 / 197, 223 / 209; CTRSV 326 / 330, 259 / 238, 260 / 293.
+// CORAL_TRSV_DEPS=<0|1|2> overrides it (complex double stays at 0).
+template <typename E> static int trsv_deps() {
+  int nd = sizeof(E) > 8 ? 0 : 1;
+  if (const char *e = getenv("CORAL_TRSV_DEPS")) nd = atoi(e);
+  return nd < 0 ? 0 : (nd > 2 ? 2 : nd);
+}
+
 template <typename E>
 static int trsv_impl(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const E *a, int64_t lda, E *x,
                      int64_t incx, coral_stream_t stream) {
@@ -545,12 +639,27 @@ static int trsv_impl(int32_t uplo, int32_t trans, int32_t diag, int64_t n, const
                      (W *)stage, nwords, Stage<E>::kSentinel, (int *)ctrl);
   CORAL_LAUNCH_CHECK();
   TriView<E> v = make_view(uplo, trans, a, lda, n);
-  if (v.trans == 0)
-    hipLaunchKernelGGL((trsv_kernel<E, 1>), dim3((unsigned)nb), dim3(256), 0, s, v, x, incx,
-                       (int)(diag == CORAL_UNIT_DIAGONAL), (W *)stage, (int *)ctrl, nb, err, wait_limit());
-  else
-    hipLaunchKernelGGL((trsv_kernel<E, 0>), dim3((unsigned)nb), dim3(256), 0, s, v, x, incx,
-                       (int)(diag == CORAL_UNIT_DIAGONAL), (W *)stage, (int *)ctrl, nb, err, wait_limit());
+  const int nd = trsv_deps<E>();
+  const int udiag = (int)(diag == CORAL_UNIT_DIAGONAL);
+  const uint32_t lim = wait_limit();
+#define CORAL_TRSV_LAUNCH(RC, ND)                                                                             \
+  hipLaunchKernelGGL((trsv_kernel<E, RC, ND>), dim3((unsigned)nb), dim3(256), 0, s, v, x, incx, udiag, (W *)stage, \
+                     (int *)ctrl, nb, err, lim)
+  if constexpr (sizeof(E) > 8) {
+    if (v.trans == 0) CORAL_TRSV_LAUNCH(1, 0);
+    else CORAL_TRSV_LAUNCH(0, 0);
+  } else {
+    if (v.trans == 0) {
+      if (nd == 2) CORAL_TRSV_LAUNCH(1, 2);
+      else if (nd == 1) CORAL_TRSV_LAUNCH(1, 1);
+      else CORAL_TRSV_LAUNCH(1, 0);
+    } else {
+      if (nd == 2) CORAL_TRSV_LAUNCH(0, 2);
+      else if (nd == 1) CORAL_TRSV_LAUNCH(0, 1);
+      else CORAL_TRSV_LAUNCH(0, 0);
+    }
+  }
+#undef CORAL_TRSV_LAUNCH
   CORAL_LAUNCH_CHECK();
   return CORAL_OK;
 }

@@ -150,6 +150,20 @@ def tri(n, lda, f, rng, uplo, cplx):
     return a
 
 
+def tri_vec(n, lda, f, rng, uplo, cplx):
+    """Same distribution as tri() (off-diagonal U[-1,1]/n, diagonal U[1,2]), drawn
+    in one vectorised pass; NaN outside the stored triangle."""
+    v = rng.uniform(-1, 1, (n, n)) / n
+    if cplx:
+        v = v + 1j * rng.uniform(-1, 1, (n, n)) / n
+    d = rng.uniform(1, 2, n) + (0.25j if cplx else 0)
+    v[np.arange(n), np.arange(n)] = d
+    keep = np.tril(np.ones((n, n), bool)) if uplo == 1 else np.triu(np.ones((n, n), bool))
+    a = np.full((n, lda), np.nan, f)  # a[j, i] = A(i, j)
+    a[:, :n] = np.where(keep.T, v.T, np.nan)
+    return a.reshape(-1)
+
+
 @pytest.mark.parametrize("p", "sdcz")
 def test_trsv_trmv_all_variants(gpu_lib, p):
     cplx = p in "cz"
@@ -178,6 +192,42 @@ def test_trsv_trmv_all_variants(gpu_lib, p):
                     # TRMV of the solution returns the right-hand side
                     getattr(blas, f"{p}trmv")(uplo, trans, diag, n, a_use, lda, xs, incx)
                     assert allclose_ref(xs, x, 1e-4 if p in "sc" else 1e-12, 1e-5 if p in "sc" else 1e-12)
+
+
+@pytest.mark.parametrize("deps", ["0", "1", "2"])
+@pytest.mark.parametrize("p", "sdc")
+def test_trsv_precomputed_dependency_depths(gpu_lib, p, deps, monkeypatch):
+    """Every depth of the precomputed block products (0: none, 1: the block just
+    above, 2: the two blocks above) over all uplo/trans/diag variants, with enough
+    64-row blocks (n from 449) that the precomputed path is taken, a ragged last
+    block, lda > n and a strided vector."""
+    monkeypatch.setenv("CORAL_TRSV_DEPS", deps)
+    cplx = p in "cz"
+    f = DT[p]
+    rng = np.random.default_rng(31)
+    eps = 2.0 ** -24 if p in "sc" else 2.0 ** -53
+    for n, lda, incx in ((449, 449, 1), (1000, 1003, 2)):
+        for uplo in (0, 1):
+            a = tri_vec(n, lda, f, rng, uplo, cplx)
+            for trans in (0, 1, 2):
+                for diag in (0, 1):
+                    a_use = a.copy()
+                    if diag == 0:
+                        a_use[np.arange(n) * (lda + 1)] = np.nan
+                    x = rand(rng, (n - 1) * incx + 1, f)
+                    xs = x.copy()
+                    getattr(blas, f"{p}trsv")(uplo, trans, diag, n, a_use, lda, xs, incx)
+                    M = ref64.triangular(a_use, n, lda, uplo, trans, diag)
+                    xv = ref64.vec(xs, n, incx).astype(M.dtype)
+                    bv = ref64.vec(x, n, incx).astype(M.dtype)
+                    berr = np.linalg.norm(bv - M @ xv) / (np.linalg.norm(M) * np.linalg.norm(xv) + np.linalg.norm(bv))
+                    assert berr <= 4 * n * eps, (deps, n, uplo, trans, diag, berr)
+                    ref = ref64.trsv(uplo, trans, diag, n, a_use, lda, x, incx)
+                    assert allclose_ref(xs, ref, 1e-4 if p in "sc" else 1e-12, 1e-5 if p in "sc" else 1e-12)
+                    # the strided vector's gaps are left alone
+                    gaps = np.ones(len(x), bool)
+                    gaps[::incx] = False
+                    assert np.array_equal(xs[gaps], x[gaps])
 
 
 def _tri_fast(n, lda, f, rng, uplo, cplx):
