@@ -847,6 +847,38 @@ def _dgemv_n(gm):
             "target_frac": 0.80, "meets_target": bool(gbps / PEAK_HBM >= 0.80)}
 
 
+def _sustain(form, seconds, data, n):
+    """SGEMM NN n^3 in one form, called back to back for `seconds` s; prints the
+    throughput of each second.  data "normal": N(0,1) operands as in the headline;
+    "int": integers in [-8, 8), whose split halves have few significand bits and
+    all-zero lo halves (the same instructions, with far fewer bits toggling in the
+    multipliers).  Board power and clocks are read beside it (amd-smi metric -p -c)."""
+    from coral_amd import device, synthetic
+    from coral_amd._lib import call
+
+    A, B, C = (device.DeviceArray(n * n, np.float32) for _ in range(3))
+    if data == "normal":
+        device.fill_random(A, synthetic.seed_for(3, "A"), "normal")
+        device.fill_random(B, synthetic.seed_for(3, "B"), "normal")
+    else:
+        rng = np.random.default_rng(0)
+        A.upload(rng.integers(-8, 8, n * n).astype(np.float32))
+        B.upload(rng.integers(-8, 8, n * n).astype(np.float32))
+    fn = {"fp16x3": "coral_sgemm_fp16x3", "bf16x3": "coral_sgemm_bf16x3", "fp32": "coral_sgemm"}[form]
+    t0 = last = time.perf_counter()
+    calls = done = 0
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(10):
+            call(fn, 0, 0, n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.0, C.ptr, n, None)
+        device.synchronize()
+        calls += 10
+        now = time.perf_counter()
+        if now - last >= 1.0:
+            print(json.dumps({"sustain": form, "data": data, "n": n, "t_s": round(now - t0, 2),
+                              "tflops": round(2.0 * n ** 3 * (calls - done) / (now - last) / 1e12, 1)}), flush=True)
+            last, done = now, calls
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -861,6 +893,11 @@ def main():
                          "without the per-problem loop)")
     ap.add_argument("--suite", default="",
                     help="comma list of c1,c2,c3,c4,c5,rows (or one part of rows: gemv,symv,tr,rank,l1): one JSON line per configuration")
+    ap.add_argument("--sustain", type=float, default=0.0,
+                    help="seconds of back-to-back SGEMM n^3 calls in --sustain-form on --sustain-data "
+                         "(throughput per second; for reading board power and clocks beside it)")
+    ap.add_argument("--sustain-form", default="fp16x3", choices=("fp16x3", "bf16x3", "fp32"))
+    ap.add_argument("--sustain-data", default="normal", choices=("normal", "int"))
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -880,6 +917,9 @@ def main():
     if st != 0:
         raise SystemExit(f"coral_init failed: {lib.coral_last_error().decode()}")
     log("device:", device.device_name())
+    if args.sustain > 0:
+        _sustain(args.sustain_form, args.sustain, args.sustain_data, args.n)
+        return
 
     if "split" not in parts:
         # a partial line (profiling runs): the named parts only, no headline value

@@ -786,7 +786,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 #define CORAL_LDS_PTR(p) ((__attribute__((address_space(3))) void *)(p))
 
 // Direct-to-LDS kernel: WVM x WVN waves; packed tiles go HBM -> LDS with
-// global_load_lds_dwordx4 (lane l of a wave writes 16 bytes at the wave's base +
+// buffer_load_dwordx4 ... lds (lane l of a wave writes 16 bytes at the wave's base +
 // 16 l, which is exactly the fragment order of the packed tile), so no registers
 // carry tiles in flight.  Two LDS stages in two separate arrays (the compiler can
 // then tell a ds_read of one stage from the DMA into the other and does not wait
@@ -834,18 +834,25 @@ __global__ __launch_bounds__(64 * WVM * WVN) void gemm_dma_kernel(GemmArgs g) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
 
+  // Buffer-to-LDS DMA through a descriptor over the packed tile.  Not global_load_lds:
+  // the compiler books that FLAT-encoded form as a possible LDS access on the LGKM
+  // counter, which makes it treat that counter as out of order and wait for every
+  // outstanding fragment read (lgkmcnt(0)) before each MFMA group; with buffer loads
+  // it waits for exactly the reads each MFMA needs.
   auto dma = [&](v4u *st, int64_t t) {
-    const v4u *an = Ag + t * ATV;
-    const v4u *bn = Bg + t * BTV;
-    const uint32_t lane_off = (uint32_t)tid * 16u;
+    const __amdgpu_buffer_rsrc_t ra =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(Ag + t * ATV), (short)0, ATV * 16, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rb =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(Bg + t * BTV), (short)0, BTV * 16, 0x00020000);
+    const int32_t lane_off = tid * 16;
 #pragma unroll
     for (int i = 0; i < LA; ++i)
-      __builtin_amdgcn_global_load_lds((const void *)((const char *)(an + i * NT) + lane_off),
-                                       CORAL_LDS_PTR(st + i * NT + wbase), 16, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, CORAL_LDS_PTR(st + i * NT + wbase), 16, lane_off + i * NT * 16, 0,
+                                               0, 0);
 #pragma unroll
     for (int i = 0; i < LB; ++i)
-      __builtin_amdgcn_global_load_lds((const void *)((const char *)(bn + i * NT) + lane_off),
-                                       CORAL_LDS_PTR(st + ATV + i * NT + wbase), 16, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, CORAL_LDS_PTR(st + ATV + i * NT + wbase), 16,
+                                               lane_off + i * NT * 16, 0, 0, 0);
   };
   auto compute = [&](const v4u *st) {
     const v4u *la = st;
