@@ -929,32 +929,43 @@ __global__ __launch_bounds__(64 * WVM * WVN) void gemm_dma_kernel(GemmArgs g) {
 // tile of C reads op(A) and op(B) straight from the caller's arrays (coalesced along
 // whichever index is contiguous), writes them to LDS in fragment order and runs the
 // MFMAs; four wavefronts each own a 16x16 block.  No pack pass: one launch.  The next
-// K step is loaded into registers while the current one is computed.
-template <class TR>
+// K step (BK deep) is loaded into registers while the current one is computed.  The
+// loads are unconditional, from indices clamped into the operand, and the elements
+// past m, n or K are zeroed when the step is written to LDS: a guarded load is a
+// branch, and at each branch join the compiler waits for every load issued so far,
+// which made a thread's loads one round trip each.
+template <class TR, int BK>
 __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, const typename TR::T *__restrict__ A,
                                                          int64_t asr, int64_t ask, const typename TR::T *__restrict__ B,
                                                          int64_t bsr, int64_t bsk, int64_t K) {
   using T = typename TR::T;
   using acc_t = typename TR::acc_t;
-  constexpr int VE = TR::VE, KF = TR::KF, SM = 32, BK = 32, KC = BK / KF;
+  constexpr int VE = TR::VE, KF = TR::KF, SM = 32, KC = BK / KF;
   constexpr int FV = KC * (SM / 16) * 64;  // fragment vectors per operand tile
   constexpr int EPT = SM * BK / 256;       // elements per thread per operand
   static_assert(!TR::SPLIT && TR::PL == 1, "plain MFMA types only");
+  static_assert(BK % KF == 0 && EPT >= 1, "k step must hold whole fragments");
   __shared__ v4u la[FV], lb[FV];
   T *las = reinterpret_cast<T *>(la);
   T *lbs = reinterpret_cast<T *>(lb);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, wm = wave & 1, wn = wave >> 1;
   const int64_t m0 = (int64_t)blockIdx.x * SM, n0 = (int64_t)blockIdx.y * SM;
+  // element e of a step: (row, k) of the 32 x BK block, rows or k fastest, whichever
+  // is contiguous in the operand
+  auto rk = [](bool rows_contig, int e, int &r, int &kk) {
+    r = rows_contig ? e % SM : e / BK;
+    kk = rows_contig ? e / SM : e % BK;
+  };
   T va[EPT], vb[EPT];
   auto gather = [&](int64_t k0) {
 #pragma unroll
     for (int i = 0; i < EPT; ++i) {
       const int e = t + 256 * i;
-      int r = asr == 1 ? e % SM : e / BK, kk = asr == 1 ? e / SM : e % BK;
-      va[i] = (m0 + r < g.M && k0 + kk < K) ? A[(m0 + r) * asr + (k0 + kk) * ask] : T(0);
-      r = bsr == 1 ? e % SM : e / BK;
-      kk = bsr == 1 ? e / SM : e % BK;
-      vb[i] = (n0 + r < g.N && k0 + kk < K) ? B[(n0 + r) * bsr + (k0 + kk) * bsk] : T(0);
+      int r, kk;
+      rk(asr == 1, e, r, kk);
+      va[i] = A[imin64(m0 + r, g.M - 1) * asr + imin64(k0 + kk, K - 1) * ask];
+      rk(bsr == 1, e, r, kk);
+      vb[i] = B[imin64(n0 + r, g.N - 1) * bsr + imin64(k0 + kk, K - 1) * bsk];
     }
   };
   auto frag_pos = [](int r, int kk) {  // element offset of (row r, k kk) in the fragment-ordered tile
@@ -967,8 +978,11 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, const typen
 #pragma unroll
     for (int i = 0; i < EPT; ++i) {
       const int e = t + 256 * i;
-      las[asr == 1 ? frag_pos(e % SM, e / SM) : frag_pos(e / BK, e % BK)] = va[i];
-      lbs[bsr == 1 ? frag_pos(e % SM, e / SM) : frag_pos(e / BK, e % BK)] = vb[i];
+      int r, kk;
+      rk(asr == 1, e, r, kk);
+      las[frag_pos(r, kk)] = (m0 + r < g.M && k0 + kk < K) ? va[i] : T(0);
+      rk(bsr == 1, e, r, kk);
+      lbs[frag_pos(r, kk)] = (n0 + r < g.N && k0 + kk < K) ? vb[i] : T(0);
     }
     __syncthreads();
     if (k0 + BK < K) gather(k0 + BK);
@@ -1405,6 +1419,18 @@ static int quick_scale(T *c, int64_t m, int64_t n, int64_t ldc, double br, doubl
   return CORAL_OK;
 }
 
+// k step of gemm_small_kernel: 32 for K up to 32, else 64 (SGEMM 512^3 with
+// alpha = 1, beta = 0.5: 14.0 / 12.9 / 13.5 us per call with 32 / 64 / 128).
+// CORAL_SMALL_BK = 32 or 64 overrides, for measurement and tests.
+static int small_bk(int64_t k) {
+  static const int forced = [] {
+    const char *e = getenv("CORAL_SMALL_BK");
+    return e ? atoi(e) : 0;
+  }();
+  if (forced == 32 || forced == 64) return forced;
+  return k <= 32 ? 32 : 64;
+}
+
 template <class TR, typename T>
 static int real_gemm(const char *routine, int32_t op_a, int32_t op_b, int64_t m, int64_t n, int64_t k, T alpha,
                      const T *a, int64_t lda, const T *b, int64_t ldb, T beta, T *c, int64_t ldc,
@@ -1431,8 +1457,12 @@ static int real_gemm(const char *routine, int32_t op_a, int32_t op_b, int64_t m,
       g.br = (double)beta;
       g.beta_zero = beta == T(0);
       const int slot = timing_begin(TK_GEMM, s);
-      hipLaunchKernelGGL(gemm_small_kernel<TR>, dim3((unsigned)cdiv(m, 32), (unsigned)cdiv(n, 32)), dim3(256), 0, s,
-                         g, a, fa.sr, fa.sk, b, fb.sr, fb.sk, k);
+      const dim3 grid((unsigned)cdiv(m, 32), (unsigned)cdiv(n, 32));
+      const int bk = small_bk(k);
+      if (bk == 32)
+        hipLaunchKernelGGL((gemm_small_kernel<TR, 32>), grid, dim3(256), 0, s, g, a, fa.sr, fa.sk, b, fb.sr, fb.sk, k);
+      else
+        hipLaunchKernelGGL((gemm_small_kernel<TR, 64>), grid, dim3(256), 0, s, g, a, fa.sr, fa.sk, b, fb.sr, fb.sk, k);
       CORAL_LAUNCH_CHECK();
       timing_end(slot, s);
       return CORAL_OK;

@@ -273,6 +273,8 @@ int coral_free(void *ptr) {
 int coral_memcpy_htod(void *dst, const void *src, size_t bytes) {
   CORAL_TRY(ensure_init());
   CORAL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  // from pageable memory the call can return once the data is staged for the DMA
+  CORAL_HIP(hipStreamSynchronize(nullptr));
   return CORAL_OK;
 }
 int coral_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
@@ -280,14 +282,19 @@ int coral_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
   CORAL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   return CORAL_OK;
 }
+// Device-to-device copies and fills are queued on the null stream and may return
+// before they are done; both wait for completion, so work a caller then queues on a
+// non-blocking stream (which does not wait for the null stream) sees the result.
 int coral_memcpy_dtod(void *dst, const void *src, size_t bytes) {
   CORAL_TRY(ensure_init());
   CORAL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  CORAL_HIP(hipStreamSynchronize(nullptr));
   return CORAL_OK;
 }
 int coral_memset(void *dst, int value, size_t bytes) {
   CORAL_TRY(ensure_init());
   CORAL_HIP(hipMemset(dst, value, bytes));
+  CORAL_HIP(hipStreamSynchronize(nullptr));
   return CORAL_OK;
 }
 int coral_synchronize(void) {

@@ -70,6 +70,8 @@ const char *coral_last_error(void);
 int coral_device_name(char *buf, size_t len);
 int coral_malloc(void **ptr, size_t bytes);
 int coral_free(void *ptr);
+/* The copies and the fill are blocking: they return once the device memory holds the
+ * result, so work queued afterwards on any stream sees it. */
 int coral_memcpy_htod(void *dst, const void *src, size_t bytes);
 int coral_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int coral_memcpy_dtod(void *dst, const void *src, size_t bytes);
