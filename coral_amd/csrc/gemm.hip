@@ -926,14 +926,16 @@ __global__ __launch_bounds__(64 * WVM * WVN) void gemm_dma_kernel(GemmArgs g) {
 }
 
 // Small real problems (fewer 64x64 output tiles than CUs): one workgroup per 32x32
-// tile of C reads op(A) and op(B) straight from the caller's arrays (coalesced along
-// whichever index is contiguous), writes them to LDS in fragment order and runs the
-// MFMAs; four wavefronts each own a 16x16 block.  No pack pass: one launch.  The next
-// K step (BK deep) is loaded into registers while the current one is computed.  The
-// loads are unconditional, from indices clamped into the operand, and the elements
-// past m, n or K are zeroed when the step is written to LDS: a guarded load is a
-// branch, and at each branch join the compiler waits for every load issued so far,
-// which made a thread's loads one round trip each.
+// tile of C reads op(A) and op(B) straight from the caller's arrays, writes them to
+// LDS in fragment order and runs the MFMAs; four wavefronts each own a 16x16 block.
+// No pack pass: one launch.  The next K step (BK deep) is loaded into registers while
+// the current one is computed.  Loads go by 16-byte vectors of LV elements along
+// whichever index is contiguous: one vector load when the step's block lies inside
+// the operand and the operand allows aligned vectors, else LV element loads from
+// indices clamped into the operand (the choice is uniform over the workgroup).
+// Elements past m, n or K are zeroed when the step is written to LDS.  No load sits
+// behind a per-element guard: a guarded load is a branch, and at each branch join
+// the compiler waits for every load issued so far.
 template <class TR, int BK>
 __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, const typename TR::T *__restrict__ A,
                                                          int64_t asr, int64_t ask, const typename TR::T *__restrict__ B,
@@ -941,49 +943,75 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(GemmArgs g, const typen
   using T = typename TR::T;
   using acc_t = typename TR::acc_t;
   constexpr int VE = TR::VE, KF = TR::KF, SM = 32, KC = BK / KF;
+  constexpr int LV = 16 / (int)sizeof(T);  // elements per 16-byte vector (== VE)
   constexpr int FV = KC * (SM / 16) * 64;  // fragment vectors per operand tile
-  constexpr int EPT = SM * BK / 256;       // elements per thread per operand
-  static_assert(!TR::SPLIT && TR::PL == 1, "plain MFMA types only");
-  static_assert(BK % KF == 0 && EPT >= 1, "k step must hold whole fragments");
+  constexpr int NV = SM * BK / (LV * 256); // vectors per thread per operand
+  static_assert(!TR::SPLIT && TR::PL == 1 && LV == VE, "plain MFMA types only");
+  static_assert(BK % KF == 0 && NV >= 1 && (SM * BK) % (LV * 256) == 0, "k step must hold whole fragments");
   __shared__ v4u la[FV], lb[FV];
   T *las = reinterpret_cast<T *>(la);
   T *lbs = reinterpret_cast<T *>(lb);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, wm = wave & 1, wn = wave >> 1;
   const int64_t m0 = (int64_t)blockIdx.x * SM, n0 = (int64_t)blockIdx.y * SM;
-  // element e of a step: (row, k) of the 32 x BK block, rows or k fastest, whichever
-  // is contiguous in the operand
-  auto rk = [](bool rows_contig, int e, int &r, int &kk) {
-    r = rows_contig ? e % SM : e / BK;
-    kk = rows_contig ? e / SM : e % BK;
+  const bool arows = asr == 1, brows = bsr == 1;
+  // aligned 16-byte vectors along the contiguous index (its block starts are multiples of LV)
+  const bool avec = ((uintptr_t)A % 16 == 0) && ((arows ? ask : asr) % LV == 0);
+  const bool bvec = ((uintptr_t)B % 16 == 0) && ((brows ? bsk : bsr) % LV == 0);
+  // first element (row r, k kk) of vector v of a step; the vector runs along r (rows
+  // contiguous) or along kk
+  auto vpos = [](bool rows, int v, int &r, int &kk) {
+    r = rows ? (v % (SM / LV)) * LV : v / (BK / LV);
+    kk = rows ? v / (SM / LV) : (v % (BK / LV)) * LV;
   };
-  T va[EPT], vb[EPT];
-  auto gather = [&](int64_t k0) {
+  T va[NV][LV], vb[NV][LV];
+  auto gather_one = [&](const T *X, int64_t sr, int64_t sk, bool rows, bool full, int64_t R, int64_t r0, int64_t k0,
+                        T (&vx)[NV][LV]) {
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-      const int e = t + 256 * i;
+    for (int i = 0; i < NV; ++i) {
       int r, kk;
-      rk(asr == 1, e, r, kk);
-      va[i] = A[imin64(m0 + r, g.M - 1) * asr + imin64(k0 + kk, K - 1) * ask];
-      rk(bsr == 1, e, r, kk);
-      vb[i] = B[imin64(n0 + r, g.N - 1) * bsr + imin64(k0 + kk, K - 1) * bsk];
+      vpos(rows, t + 256 * i, r, kk);
+      if (full) {
+        const v4u x = *reinterpret_cast<const v4u *>(X + (r0 + r) * sr + (k0 + kk) * sk);
+        __builtin_memcpy(vx[i], &x, 16);
+      } else {
+#pragma unroll
+        for (int e = 0; e < LV; ++e)
+          vx[i][e] = X[imin64(r0 + r + (rows ? e : 0), R - 1) * sr + imin64(k0 + kk + (rows ? 0 : e), K - 1) * sk];
+      }
     }
+  };
+  auto gather = [&](int64_t k0) {
+    const bool kin = k0 + BK <= K;
+    gather_one(A, asr, ask, arows, avec && kin && m0 + SM <= g.M, g.M, m0, k0, va);
+    gather_one(B, bsr, bsk, brows, bvec && kin && n0 + SM <= g.N, g.N, n0, k0, vb);
   };
   auto frag_pos = [](int r, int kk) {  // element offset of (row r, k kk) in the fragment-ordered tile
     return (((kk / KF) * (SM / 16) + r / 16) * 64 + (r % 16) + 16 * ((kk % KF) / VE)) * VE + kk % VE;
+  };
+  auto store_one = [&](T *ls, bool rows, int64_t R, int64_t r0, int64_t k0, const T (&vx)[NV][LV]) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      int r, kk;
+      vpos(rows, t + 256 * i, r, kk);
+      if (rows) {  // LV rows of one k: LV fragment slots
+#pragma unroll
+        for (int e = 0; e < LV; ++e) ls[frag_pos(r + e, kk)] = (r0 + r + e < R && k0 + kk < K) ? vx[i][e] : T(0);
+      } else {     // LV k values of one row: exactly one fragment vector
+        T w[LV];
+#pragma unroll
+        for (int e = 0; e < LV; ++e) w[e] = (r0 + r < R && k0 + kk + e < K) ? vx[i][e] : T(0);
+        v4u x;
+        __builtin_memcpy(&x, w, 16);
+        *reinterpret_cast<v4u *>(ls + frag_pos(r, kk)) = x;
+      }
+    }
   };
   acc_t acc = acc_t{};
   gather(0);
   for (int64_t k0 = 0; k0 < K; k0 += BK) {
     __syncthreads();  // the previous step's fragments are consumed
-#pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-      const int e = t + 256 * i;
-      int r, kk;
-      rk(asr == 1, e, r, kk);
-      las[frag_pos(r, kk)] = (m0 + r < g.M && k0 + kk < K) ? va[i] : T(0);
-      rk(bsr == 1, e, r, kk);
-      lbs[frag_pos(r, kk)] = (n0 + r < g.N && k0 + kk < K) ? vb[i] : T(0);
-    }
+    store_one(las, arows, g.M, m0, k0, va);
+    store_one(lbs, brows, g.N, n0, k0, vb);
     __syncthreads();
     if (k0 + BK < K) gather(k0 + BK);
 #pragma unroll
@@ -1420,7 +1448,7 @@ static int quick_scale(T *c, int64_t m, int64_t n, int64_t ldc, double br, doubl
 }
 
 // k step of gemm_small_kernel: 32 for K up to 32, else 64 (SGEMM 512^3 with
-// alpha = 1, beta = 0.5: 14.0 / 12.9 / 13.5 us per call with 32 / 64 / 128).
+// alpha = 1, beta = 0.5: 12.4 / 11.8 / 12.4 us per call with 32 / 64 / 128).
 // CORAL_SMALL_BK = 32 or 64 overrides, for measurement and tests.
 static int small_bk(int64_t k) {
   static const int forced = [] {
