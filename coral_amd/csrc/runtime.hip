@@ -107,10 +107,13 @@ int workspace(hipStream_t stream, int slot, size_t bytes, void **ptr) {
     void *p = nullptr;
     CORAL_HIP(hipMalloc(&p, grow));
     if (slot == WS_ERROR) {
-      hipError_t e = hipMemset(p, 0, grow);
+      // ordered before the stream's kernels that set flags in it (a plain hipMemset
+      // goes to the null stream, which a non-blocking stream does not wait for); never
+      // under capture: growth is refused there
+      hipError_t e = hipMemsetAsync(p, 0, grow, stream);
       if (e != hipSuccess) {
         (void)hipFree(p);
-        return hip_fail(e, "hipMemset", __FILE__, __LINE__);
+        return hip_fail(e, "hipMemsetAsync", __FILE__, __LINE__);
       }
     }
     b = new WsBuf;
@@ -342,7 +345,8 @@ int coral_stream_check(coral_stream_t stream) {
   int flags = 0;
   CORAL_HIP(hipMemcpy(&flags, err, sizeof(int), hipMemcpyDeviceToHost));
   if (flags == 0) return CORAL_OK;
-  CORAL_HIP(hipMemset(err, 0, sizeof(int)));
+  // cleared on the stream itself, so flags set by work queued on it later are kept
+  CORAL_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
   set_last_error("%s%s", (flags & ERRF_WAIT_BOUND) ? "a triangular solve's bounded wait gave up (its x is invalid); " : "",
                  (flags & ERRF_PROBLEM_ARG) ? "a batched call skipped problems with invalid per-problem "
                                               "dimensions or leading dimensions" : "");
