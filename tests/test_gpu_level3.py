@@ -316,3 +316,46 @@ def test_argument_errors_reported(gpu_lib):
     with pytest.raises(CoralError) as e:
         blas.sgemm(7, 0, 4, 4, 4, 1.0, a, 4, a, 4, 0.0, a, 4)
     assert e.value.status == -1
+
+
+# Problems with fewer 64x64 tiles than CUs take the one-launch small kernel: 16-byte
+# vector loads when the operand base is 16-byte aligned and its leading dimension a
+# multiple of the vector (the blocks wholly inside the operand), element loads from
+# clamped indices otherwise (edge blocks, a misaligned base).  Shapes mix full and
+# edge 32x32 tiles and full and partial 32- / 64-deep k steps.
+SMALL_SHAPES = [(64, 96, 200), (96, 33, 64), (40, 64, 130), (32, 32, 32), (128, 64, 65), (33, 31, 7)]
+
+
+@pytest.mark.parametrize("dt", ["s", "d"])
+def test_small_kernel_vector_and_element_paths(gpu_lib, dt):
+    from coral_amd import device
+    from coral_amd._lib import call
+
+    rng = np.random.default_rng(23)
+    f = DT[dt]
+    name = "coral_sgemm" if dt == "s" else "coral_dgemm"
+    worst = 0.0
+    for (m, n, k) in SMALL_SHAPES:
+        for oa in (0, 1):
+            for ob in (0, 1):
+                ar, ac = (m, k) if oa == 0 else (k, m)
+                br, bc = (k, n) if ob == 0 else (n, k)
+                lda, ldb, ldc = -(-ar // 4) * 4 + 4, -(-br // 4) * 4, m
+                a, b, c = rand(rng, lda * ac, f), rand(rng, ldb * bc, f), rand(rng, ldc * n, f)
+                ref = ref64.gemm(oa, ob, m, n, k, 0.75, a, lda, b, ldb, -0.5, c, ldc)
+                po = c.copy()
+                PORT_FN[dt](oa, ob, m, n, k, 0.75, a, lda, b, ldb, -0.5, po, ldc)
+                den = max(k, 1) * EPS[dt] * (0.75 * np.linalg.norm(ref64.mat(a, ar, ac, lda)) *
+                                            np.linalg.norm(ref64.mat(b, br, bc, ldb)) +
+                                            0.5 * np.linalg.norm(ref64.mat(c, m, n, ldc)))
+                for shift in (0, 1):  # element offset of every operand's base: 0 aligned, 1 not
+                    A, B, C = (device.DeviceArray.from_numpy(np.concatenate([np.zeros(shift, f), x]))
+                               for x in (a, b, c))
+                    es = np.dtype(f).itemsize
+                    call(name, oa, ob, m, n, k, 0.75, A.ptr + shift * es, lda, B.ptr + shift * es, ldb, -0.5,
+                         C.ptr + shift * es, ldc, None)
+                    out = C.to_numpy()[shift:]
+                    num = np.linalg.norm(ref64.mat(out, m, n, ldc).astype(np.float64) - ref64.mat(ref, m, n, ldc))
+                    worst = max(worst, num / den)
+                    assert allclose_ref(out, po, *GEMM_TOL[dt]), (m, n, k, oa, ob, shift)
+    assert worst <= 1.0, worst
