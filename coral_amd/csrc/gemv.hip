@@ -107,16 +107,20 @@ __global__ __launch_bounds__(256) void gemv_n_kernel(int64_t m, int64_t n, const
 
 // T/C: a wavefront owns CPW consecutive columns; per step lane l reads the
 // vectors at i + v*64*EV + l*EV (v < NV) of x once and of each of its columns, so
-// x is read once per CPW columns.  Either shape keeps eight 16-byte loads of A in
-// flight per lane; measured at 16384^2 / 8192^2, double-precision elements stream
-// faster with one vector of eight columns (DGEMV T 6.17 -> 6.38 TB/s, ZGEMV C
-// 5.84 -> 6.32) and single-precision ones with two vectors of four columns
-// (CGEMV C 5.94-5.96 TB/s; 5.72 with one vector of eight columns, 5.68-5.73 with
-// two of eight, 5.88-5.90 with four of four, 5.57-5.62 with four of two).
+// x is read once per CPW columns.  Measured at 16384^2 / 8192^2: double-precision
+// elements stream faster with one vector of eight columns (DGEMV T 6.17 -> 6.38
+// TB/s, ZGEMV C 5.84 -> 6.32; ZGEMV C 5.90-5.95 with sixteen columns or one vector
+// of four), complex float with two vectors of four columns (CGEMV C 5.94-5.96 TB/s;
+// 5.72 with one vector of eight columns, 5.68-5.73 with two of eight, 5.58 with one
+// of sixteen, 5.88-5.90 with four of four, 5.57-5.62 with four of two), real float
+// with one vector of sixteen columns (SGEMV T 6.20-6.25 -> 6.68 TB/s; 6.37 with one
+// of eight, 6.39 with one of thirty-two, 6.24 with two of eight, 6.69 with two of
+// sixteen).
 template <typename E>
 struct GemvTShape {
   static constexpr bool kDouble = std::is_same<typename elem_traits<E>::real, double>::value;
-  static constexpr int NV = kDouble ? 1 : 2, CPW = kDouble ? 8 : 4;
+  static constexpr bool kReal32 = std::is_same<E, float>::value;
+  static constexpr int NV = kDouble ? 1 : (kReal32 ? 1 : 2), CPW = kDouble ? 8 : (kReal32 ? 16 : 4);
 };
 template <typename E, bool CONJ>
 __global__ __launch_bounds__(256) void gemv_t_kernel(int64_t m, int64_t n, const E *__restrict__ A, int64_t lda,
