@@ -71,6 +71,23 @@ def timed_steps(fn, steps, warmup):
     return e0.elapsed_ms(e1) / steps
 
 
+def timed_with_kernel(fn, steps, kind):
+    """Whole-call ms per step over `steps` calls with the library's kernel timing off
+    (its event pair around each launch adds a gap to every call), then the main
+    kernel's mean duration over another `steps` calls with it on.  Returns
+    (call_ms, kernel_ms, kernel_launches)."""
+    from coral_amd._lib import call
+
+    ms = timed_steps(fn, steps, 0)
+    call("coral_timing_reset")
+    call("coral_timing_enable", 1)
+    timed_steps(fn, steps, 0)
+    k_ms, k_cnt = kernel_avg_ms(kind)
+    call("coral_timing_enable", 0)
+    call("coral_timing_reset")
+    return ms, k_ms, k_cnt
+
+
 def kernel_avg_ms(kind):
     import ctypes
 
@@ -219,12 +236,8 @@ def suite_c2():
     device.fill_random(B, synthetic.seed_for(2, "B"), "normal")
     for ops in ((0, 0), (1, 1)):
         step = lambda: call("coral_dgemm", ops[0], ops[1], n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.0, C.ptr, n, None)
-        timed_steps(step, 3, 0)  # warm clocks; kernel timing covers the timed calls only
-        call("coral_timing_reset")
-        call("coral_timing_enable", 1)
-        ms = timed_steps(step, 5, 0)
-        k_ms, _ = kernel_avg_ms(0)
-        call("coral_timing_enable", 0)
+        timed_steps(step, 3, 0)  # warm clocks
+        ms, k_ms, _ = timed_with_kernel(step, 5, 0)
         Ah, Bh, Ch = (_host_mat(x, n, n, np.float64) for x in (A, B, C))
         rows = np.random.default_rng(2).choice(n, 64, replace=False)
         err = _fro_sample_err(Ah, Bh, Ch, rows, n, eps=2.0 ** -53, ops=ops)
@@ -248,11 +261,7 @@ def suite_c2():
     for tr in (0, 1):
         step = lambda: call("coral_dgemv", tr, m, m, 1.0, GA.ptr, m, gx.ptr, 1, 0.0, gy.ptr, 1, None)
         timed_steps(step, 5, 0)
-        call("coral_timing_reset")
-        call("coral_timing_enable", 1)
-        ms = timed_steps(step, 20, 0)
-        k_ms, _ = kernel_avg_ms(1)
-        call("coral_timing_enable", 0)
+        ms, k_ms, _ = timed_with_kernel(step, 20, 1)
         by = 8.0 * (m * m + 2 * m)
         mc = 8192  # bounded CPU sample (512 MiB matrix)
         if tr == 0:
@@ -771,12 +780,7 @@ def _split_forms(n, steps, warmup):
         for _ in range(warmup):
             step()
         device.synchronize()
-        call("coral_timing_reset")
-        call("coral_timing_enable", 1)
-        ms = timed_steps(step, steps, 0)
-        k_ms, k_cnt = kernel_avg_ms(0)
-        call("coral_timing_enable", 0)
-        call("coral_timing_reset")
+        ms, k_ms, k_cnt = timed_with_kernel(step, steps, 0)
         res[form] = {"ms": ms, "kernel_ms": k_ms, "kernel_launches": k_cnt, "out": C}
         log(f"{form} sgemm {n}^3: {ms:.3f} ms/step -> {2.0 * n ** 3 / ms / 1e9:.1f} TFLOP/s; "
             f"MFMA kernel {k_ms:.3f} ms x {k_cnt}")
@@ -808,12 +812,7 @@ def _dgemm_nn(n, steps):
     step = lambda: call("coral_dgemm", 0, 0, n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.0, C.ptr, n, None)
     step()
     device.synchronize()
-    call("coral_timing_reset")
-    call("coral_timing_enable", 1)
-    ms = timed_steps(step, steps, 0)
-    k_ms, _ = kernel_avg_ms(0)
-    call("coral_timing_enable", 0)
-    call("coral_timing_reset")
+    ms, k_ms, _ = timed_with_kernel(step, steps, 0)
     tf = 2.0 * n ** 3 / ms / 1e9
     log(f"dgemm NN {n}^3: {ms:.3f} ms/step -> {tf:.1f} TFLOP/s")
     return {"tflops": round(tf, 2), "frac_of_fp64_peak": round(tf / PEAK_FP64, 4),
@@ -833,12 +832,8 @@ def _dgemv_n(gm):
     device.fill_random(gy, synthetic.seed_for(2, "y"), "uniform")
     step = lambda: call("coral_dgemv", 0, gm, gm, 1.0, GA.ptr, gm, gx.ptr, 1, 0.0, gy.ptr, 1, None)
     step()
-    device.synchronize()
-    call("coral_timing_enable", 1)
-    gms = timed_steps(step, 20, 2)
-    gk_ms, _ = kernel_avg_ms(1)
-    call("coral_timing_enable", 0)
-    call("coral_timing_reset")
+    timed_steps(step, 2, 0)
+    gms, gk_ms, _ = timed_with_kernel(step, 20, 1)
     gbps = 8.0 * (gm * gm + gm + gm) / gms / 1e6
     kgbps = 8.0 * (gm * gm + gm) / gk_ms / 1e6
     log(f"dgemv N {gm}^2: {gms * 1e3:.1f} us/call -> {gbps:.0f} GB/s (stream kernel {kgbps:.0f} GB/s)")
