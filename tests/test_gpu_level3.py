@@ -359,3 +359,30 @@ def test_small_kernel_vector_and_element_paths(gpu_lib, dt):
                     worst = max(worst, num / den)
                     assert allclose_ref(out, po, *GEMM_TOL[dt]), (m, n, k, oa, ob, shift)
     assert worst <= 1.0, worst
+
+
+@pytest.mark.parametrize("which", ["sgemm", "sgemm_bf16x3", "sgemm_fp16x3"])
+def test_non_finite_operands_stay_in_their_row_and_column(gpu_lib, which):
+    """Inf in A(5, 7) and NaN in B(3, 9): row 5 and column 9 of C are not finite
+    (plain FP32 gives +-Inf in row 5; the split forms may give NaN there, since a cross
+    product multiplies the Inf by a lo half that can be zero), every other element is
+    finite and within the form's normwise bound of FP64.  The row and column scaling of
+    the split forms must not carry the special value into other rows or columns."""
+    rng = np.random.default_rng(31)
+    m = n = k = 160
+    a = rng.uniform(-1, 1, m * k).astype(np.float32)
+    b = rng.uniform(-1, 1, k * n).astype(np.float32)
+    a[5 + 7 * m] = np.inf
+    b[3 + 9 * k] = np.nan
+    c = np.zeros(m * n, np.float32)
+    getattr(blas, which)(0, 0, m, n, k, 1.0, a, m, b, k, 0.0, c, m)
+    C = c.reshape(n, m).T
+    assert not np.isfinite(C[5, :]).any() and np.isnan(C[:, 9]).all()
+    rows, cols = np.arange(m) != 5, np.arange(n) != 9
+    A = a.reshape(k, m).T.astype(np.float64)[rows]
+    B = b.reshape(n, k).T.astype(np.float64)[:, cols]
+    got = C[rows][:, cols].astype(np.float64)
+    assert np.isfinite(got).all()
+    eps = {"sgemm": k * 2.0 ** -24, "sgemm_bf16x3": 2.0 ** -16 + k * 2.0 ** -24,
+           "sgemm_fp16x3": 2.0 ** -21 + k * 2.0 ** -24}[which]
+    assert np.linalg.norm(got - A @ B) <= eps * np.linalg.norm(A) * np.linalg.norm(B)
