@@ -842,6 +842,155 @@ def _dgemv_n(gm):
             "target_frac": 0.80, "meets_target": bool(gbps / PEAK_HBM >= 0.80)}
 
 
+def _other_configs():
+    """The benchmark configurations the headline line does not already carry, timed
+    compactly (no CPU legs; `--suite c1,...,c5` has the full lines): C3 SGEMM at 4096
+    and 16384 and the wide distribution, C1 SGEMM 512^3, C2 DTRSV 8192, C4 batched
+    DTRSV, C5 CGEMM CC 4096^3 (4M and both 3M forms), ZGEMM NN 4096^3 and CHEMV upper
+    16384, each with its check against FP64."""
+    import ctypes
+
+    from coral_amd import device, synthetic
+    from coral_amd._lib import call
+
+    out = {}
+    # C3: the three SGEMM forms at the sizes and distributions the headline does not
+    # cover (N(0,1) at 8192 is the headline), error from a 32-row sample
+    for n, dist in ((4096, "normal"), (4096, "wide"), (8192, "wide"), (16384, "normal"), (16384, "wide")):
+        A, B, C = (device.DeviceArray(n * n, np.float32) for _ in range(3))
+        device.fill_random(A, synthetic.seed_for(3, "A") + n, "normal")
+        device.fill_random(B, synthetic.seed_for(3, "B") + n, "normal")
+        if dist == "wide":  # row i of A and column j of B times 2^e, e uniform in [-20, 20]
+            rng = np.random.default_rng(n)
+            ea = device.DeviceArray.from_numpy(rng.integers(-20, 21, n).astype(np.int32))
+            eb = device.DeviceArray.from_numpy(rng.integers(-20, 21, n).astype(np.int32))
+            call("coral_scale_pow2_f32", A.ptr, n, n, n, ea.ptr, None, None)
+            call("coral_scale_pow2_f32", B.ptr, n, n, n, None, eb.ptr, None)
+        Ah, Bh = _host_mat(A, n, n, np.float32), _host_mat(B, n, n, np.float32)
+        rows = np.random.default_rng(5).choice(n, 32, replace=False)
+        r = {}
+        for form, fn in (("plain_fp32", "coral_sgemm"), ("bf16x3", "coral_sgemm_bf16x3"), ("fp16x3", "coral_sgemm_fp16x3")):
+            ms = timed_steps(lambda: call(fn, 0, 0, n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.0, C.ptr, n, None),
+                             2 if n >= 16384 else 5, 2 if n >= 16384 else 5)
+            err = _fro_sample_err(Ah, Bh, _host_mat(C, n, n, np.float32), rows, n)
+            r[form] = {"tflops": round(2 * n ** 3 / ms / 1e9, 2), "normwise_bound_ratio": err["normwise_bound_ratio"]}
+        for form in ("bf16x3", "fp16x3"):
+            r[form]["over_plain"] = round(r[form]["tflops"] / r["plain_fp32"]["tflops"], 3)
+            r[form]["meets_error_condition"] = bool(r[form]["normwise_bound_ratio"] <=
+                                                    r["plain_fp32"]["normwise_bound_ratio"])
+        out[f"c3_sgemm_{n}_{dist}"] = r
+        del A, B, C, Ah, Bh
+    # C1: SGEMM NN 512^3, alpha = 1, beta = 0.5, U[-1,1]
+    n = 512
+    rng = np.random.default_rng(100)
+    a, b, c = (rng.uniform(-1, 1, n * n).astype(np.float32) for _ in range(3))
+    A, B, C = (device.DeviceArray.from_numpy(x) for x in (a, b, c))
+    ms = timed_steps(lambda: call("coral_sgemm", 0, 0, n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.5, C.ptr, n, None), 50, 5)
+    C.upload(c)
+    call("coral_sgemm", 0, 0, n, n, n, 1.0, A.ptr, n, B.ptr, n, 0.5, C.ptr, n, None)
+    ref = a.reshape(n, n).T.astype(np.float64) @ b.reshape(n, n).T.astype(np.float64) + 0.5 * c.reshape(n, n).T
+    err = np.linalg.norm(C.to_numpy().reshape(n, n).T - ref) / (
+        n * 2.0 ** -24 * np.linalg.norm(a) * np.linalg.norm(b))
+    out["c1_sgemm_512_alpha1_beta05"] = {"us": round(ms * 1e3, 2), "tflops": round(2 * n ** 3 / ms / 1e9, 2),
+                                         "normwise_bound_ratio": float(err)}
+    del A, B, C
+    # C2: DTRSV lower non-unit 8192, strictly lower U[-1,1]/n, diagonal U[1,2]
+    n = 8192
+    rng = np.random.default_rng(3)
+    Ah = np.tril(rng.uniform(-1, 1, (n, n)) / n, -1) + np.diag(rng.uniform(1, 2, n))
+    bh = rng.uniform(-1, 1, n)
+    A = device.DeviceArray.from_numpy(np.asfortranarray(Ah).reshape(-1, order="F"))
+    X = device.DeviceArray.from_numpy(bh)
+    e0, e1 = device.Event(), device.Event()
+    times = []
+    for _ in range(6):
+        X.upload(bh)
+        device.synchronize()
+        e0.record()
+        call("coral_dtrsv", 1, 0, 1, n, A.ptr, n, X.ptr, 1, None)
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_ms(e1))
+    x = X.to_numpy()
+    berr = np.linalg.norm(bh - Ah @ x) / (np.linalg.norm(Ah) * np.linalg.norm(x) + np.linalg.norm(bh))
+    ms = statistics.median(times[1:])
+    out["c2_dtrsv_lower_8192"] = {"us": round(ms * 1e3, 1), "gbps": round(8.0 * (n * (n + 1) / 2 + 2 * n) / ms / 1e6, 1),
+                                  "backward_error": float(berr)}
+    del A, X, Ah
+    # C4: 4096 batched DTRSV lower non-unit, n log-U[32, 256], against a per-problem loop
+    rng = np.random.default_rng(4)
+    ns = np.exp(rng.uniform(np.log(32), np.log(257), 4096)).astype(np.int64).clip(32, 256)
+    mats = [np.asfortranarray(np.diag(rng.uniform(1, 2, k)) + np.tril(rng.uniform(-1, 1, (k, k)) / k, -1))
+            .reshape(-1, order="F") for k in ns]
+    xs = [rng.uniform(-1, 1, k) for k in ns]
+    oa = np.concatenate([[0], np.cumsum([m.size for m in mats])[:-1]])
+    ox = np.concatenate([[0], np.cumsum([v.size for v in xs])[:-1]])
+    DA = device.DeviceArray.from_numpy(np.concatenate(mats))
+    b0 = np.concatenate(xs)
+    DX = device.DeviceArray.from_numpy(b0)
+    PA = device.DeviceArray.from_numpy((DA.ptr + 8 * oa).astype(np.uint64))
+    PX = device.DeviceArray.from_numpy((DX.ptr + 8 * ox).astype(np.uint64))
+    NS = device.DeviceArray.from_numpy(ns.astype(np.int32))
+    times = []
+    for _ in range(5):
+        DX.upload(b0)
+        device.synchronize()
+        e0.record()
+        call("coral_dtrsv_batched", 1, 0, 1, len(ns), NS.ptr, PA.ptr, NS.ptr, PX.ptr, None)
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_ms(e1))
+    xb = DX.to_numpy()
+    worst = max(np.linalg.norm(xs[i] - np.tril(mats[i].reshape(ns[i], ns[i], order="F")) @ xb[ox[i]:ox[i] + ns[i]])
+                / (np.linalg.norm(mats[i]) * np.linalg.norm(xb[ox[i]:ox[i] + ns[i]]) + np.linalg.norm(xs[i]))
+                for i in range(0, len(ns), 64))
+    ms_b = statistics.median(times[1:])
+    DX.upload(b0)
+    device.synchronize()
+    e0.record()
+    for k, a_off, x_off in zip(ns, oa, ox):
+        call("coral_dtrsv", 1, 0, 1, int(k), DA.ptr + 8 * int(a_off), int(k), DX.ptr + 8 * int(x_off), 1, None)
+    e1.record()
+    e1.synchronize()
+    ms_l = e0.elapsed_ms(e1)
+    out["c4_batched_dtrsv_4096"] = {"ms": round(ms_b, 3), "loop_ms": round(ms_l, 2), "speedup": round(ms_l / ms_b, 1),
+                                    "sampled_backward_error_max": float(worst)}
+    del DA, DX, PA, PX, NS
+    # C5: CGEMM CC 4096^3 (4M FP32 MFMA, 3M BF16x3, 3M FP16x3), ZGEMM NN 4096^3, CHEMV upper 16384
+    n = 4096
+    A, B, C = (device.DeviceArray(n * n, np.complex64) for _ in range(3))
+    device.fill_random(A, synthetic.seed_for(5, "A"), "uniform")
+    device.fill_random(B, synthetic.seed_for(5, "B"), "uniform")
+    one, zero = (ctypes.c_float * 2)(1.0, 0.0), (ctypes.c_float * 2)(0.0, 0.0)
+    Ah, Bh = _host_mat(A, n, n, np.complex64), _host_mat(B, n, n, np.complex64)
+    rows = np.random.default_rng(6).choice(n, 32, replace=False)
+    for name, key in (("coral_cgemm", "c5_cgemm_cc_4096_4m_fp32"), ("coral_cgemm_3m_bf16x3", "c5_cgemm_cc_4096_3m_bf16x3"),
+                      ("coral_cgemm_3m_fp16x3", "c5_cgemm_cc_4096_3m_fp16x3")):
+        ms = timed_steps(lambda: call(name, 2, 2, n, n, n, one, A.ptr, n, B.ptr, n, zero, C.ptr, n, None), 5, 2)
+        err = _fro_sample_err(Ah, Bh, _host_mat(C, n, n, np.complex64), rows, n, cplx=True, ops=(2, 2))
+        out[key] = {"ms": round(ms, 3), "tflops": round(8 * n ** 3 / ms / 1e9, 2),
+                    "normwise_bound_ratio": err["normwise_bound_ratio"]}
+    del A, B, C, Ah, Bh
+    A, B, C = (device.DeviceArray(n * n, np.complex128) for _ in range(3))
+    device.fill_random(A, synthetic.seed_for(5, "A"), "uniform")
+    device.fill_random(B, synthetic.seed_for(5, "B"), "uniform")
+    oned, zerod = (ctypes.c_double * 2)(1.0, 0.0), (ctypes.c_double * 2)(0.0, 0.0)
+    ms = timed_steps(lambda: call("coral_zgemm", 0, 0, n, n, n, oned, A.ptr, n, B.ptr, n, zerod, C.ptr, n, None), 3, 1)
+    tf = 8 * n ** 3 / ms / 1e9
+    out["c5_zgemm_nn_4096"] = {"ms": round(ms, 3), "tflops": round(tf, 2), "frac_of_fp64_peak": round(tf / PEAK_FP64, 4)}
+    del A, B, C
+    m = 16384
+    Mh = device.DeviceArray(m * m, np.complex64)
+    device.fill_random(Mh, synthetic.seed_for(5, "C"), "uniform")
+    x, y = device.DeviceArray(m, np.complex64), device.DeviceArray(m, np.complex64)
+    device.fill_random(x, synthetic.seed_for(5, "x"), "uniform")
+    ms = timed_steps(lambda: call("coral_chemv", 0, m, one, Mh.ptr, m, x.ptr, 1, zero, y.ptr, 1, None), 10, 2)
+    by = 8.0 * m * (m + 1) / 2 + 3 * 8.0 * m
+    out["c5_chemv_upper_16384"] = {"us": round(ms * 1e3, 1), "gbps": round(by / ms / 1e6, 1),
+                                   "frac_of_hbm": round(by / ms / 1e6 / PEAK_HBM, 4)}
+    return out
+
+
 def _sustain(form, seconds, data, n):
     """SGEMM NN n^3 in one form, called back to back for `seconds` s; prints the
     throughput of each second.  data "normal": N(0,1) operands as in the headline;
@@ -882,10 +1031,10 @@ def main():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--gemv-n", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--parts", default="split,dgemm,dgemv,batched",
+    ap.add_argument("--parts", default="split,dgemm,dgemv,batched,configs",
                     help="comma list of the headline line's parts: split (the headline value; without it a "
                          "partial line is printed), dgemm, dgemv, batched (or batched-noloop: the graph replay "
-                         "without the per-problem loop)")
+                         "without the per-problem loop), configs (the other benchmark configurations, compact)")
     ap.add_argument("--suite", default="",
                     help="comma list of c1,c2,c3,c4,c5,rows (or one part of rows: gemv,symv,tr,rank,l1): one JSON line per configuration")
     ap.add_argument("--sustain", type=float, default=0.0,
@@ -1012,6 +1161,9 @@ def main():
                                                       "N(0,1), one hipGraph replay vs a per-problem launch loop",
                                           "tile_table": os.path.relpath(table, ROOT) if table else None,
                                           "target_speedup": 5.0, **bl}
+    if "configs" in parts:
+        log("other configurations ...")
+        line["other_configs"] = _other_configs()
     if not args.no_cpu_baseline:
         log("cpu baseline (CORAL SGEMM restatement, 1 core) ...")
         line["cpu_baseline"] = cpu_baseline()
