@@ -1077,6 +1077,8 @@ def main():
                 r = batched_case(dt, loop="batched" in parts)
                 r.pop("_dims")
                 line[f"batched_{key}"] = r
+        if "configs" in parts:
+            line["other_configs"] = _other_configs()
         print(json.dumps(line), flush=True)
         return
 
