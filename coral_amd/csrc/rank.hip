@@ -7,7 +7,7 @@
 // ssyr2.rs:59-197, cher.rs:62-212, cher2.rs:65-311 and the d/z twins).
 //
 // Here one launch streams the stored part of A once, read-modify-write: a
-// workgroup owns 256*EV rows x 16 columns (EV elements per 16-byte vector),
+// workgroup owns 256*EV rows x 4, 8 or 16 columns (by kind; EV elements per 16-byte vector),
 // keeps its rows of x (and y) in registers and has four columns of 16-byte
 // nontemporal loads in flight per thread.  Every wave instruction covers one
 // contiguous 1 KiB run of a column.
@@ -37,7 +37,6 @@ enum Mode { M_PLAIN = 0, M_UNIT = 1, M_FUSED = 2 };
 
 constexpr int kThreads = 256;
 constexpr int kU = 4;      // columns in flight per thread
-constexpr int kCols = 16;  // columns per workgroup
 
 template <typename E>
 struct RankArgs {
@@ -408,23 +407,27 @@ int rank_impl(const ArgPos &ap, int32_t uplo, int64_t m, int64_t n, E alpha, con
   constexpr int EV = 16 / sizeof(E);
   constexpr int64_t RB = (int64_t)kThreads * EV;
   const int64_t row_blocks = cdiv(m, RB);
-  // 16 columns per workgroup; measured on 16384^2 DGER: 16 columns 6.05 TB/s,
-  // 32 columns 4.96 TB/s, 64 columns 4.75 TB/s (8 and 4 columns within 1%).
+  // Columns per workgroup, per kind.  Measured (TB/s, 16384^2 real / 8192^2 complex,
+  // columns 16 -> 8 -> 4): SGER 6.28 -> 6.33 -> 6.61, DGER 6.01 -> 6.21 -> 6.20,
+  // CGERU 6.16 -> 6.24 -> 6.60, ZGERC 5.76 -> 6.16 -> 6.34, CHER 4.95 -> 5.92 -> 6.19,
+  // DSYR 5.61 -> 5.73 -> 5.32, ZHER2 5.18 -> 5.41-5.50 -> 5.03, DSYR2 5.26 -> 5.08 ->
+  // 4.61; 32 columns were slower for every kind (DGER 4.68).
+  constexpr int64_t cols = K == K_SYR2 ? 16 : (K == K_SYR || K == K_HER2) ? 8 : 4;
   dim3 grid;
   if (TRI) {
-    p.cpc = kCols;  // divides RB, as the tile enumeration needs
-    const int64_t KT = RB / kCols;
+    p.cpc = cols;  // divides RB, as the tile enumeration needs
+    const int64_t KT = RB / cols;
     int64_t tiles;
     if (p.lower) {
       tiles = KT * (row_blocks * (row_blocks + 1) / 2);
     } else {
-      const int64_t groups = cdiv(cdiv(n, (int64_t)kCols), KT);
+      const int64_t groups = cdiv(cdiv(n, cols), KT);
       tiles = KT * (groups * (groups + 1) / 2);
     }
     if (tiles > 0x7fffffff) return CORAL_ERR_UNSUPPORTED;
     grid = dim3((unsigned)tiles);
   } else {
-    p.cpc = std::max<int64_t>(kCols, cdiv(n, 65535));
+    p.cpc = std::max<int64_t>(cols, cdiv(n, 65535));
     if (row_blocks > 0x7fffffff) return CORAL_ERR_UNSUPPORTED;
     grid = dim3((unsigned)row_blocks, (unsigned)cdiv(n, p.cpc));
   }
